@@ -1,0 +1,191 @@
+"""NRC hot-path benchmark on one MI355X.
+
+Workload (single cache, FP16 MFMA vs FP32 plain): per frame, inference over
+2,073,600 query records (position U[-1,1]^3, direction and normal uniform on the
+sphere, roughness, albedos; SINCOS6 encoding) followed by 4 Adam training steps
+of 16,384 records each (relative-L2 loss, dynamic loss scale starting at 128),
+the network 64 -> 5 x 64 -> 3.  A step is one frame.  Inputs are resident in HBM
+before the timed region.
+
+Prints one JSON line.  value = queries per second of whole frames (inference
+plus the 4 training steps), measured between two device synchronisations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+QUERIES = 2073600
+BATCH = 16384
+BATCHES = 4
+INFER_FLOP_PER_QUERY = 41344      # 2 * (5 * 64 * 64 + 3 * 64)
+TRAIN_FLOP_PER_SAMPLE = 115840    # forward 41,344 + dW 41,344 + dA 33,152
+QUERY_BYTES = 56 + 12             # 14-float record in, RGB out
+PEAK_F16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> dict:
+    """The FP32 CPU port of the same MLP (oracle/, REAL=float), on a bounded sample."""
+    from oracle.oracle import Oracle
+    o = Oracle("f32")
+    try:
+        cores = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        cores = min(16, os.cpu_count() or 1)
+    o.set_threads(cores)
+    from vknrc_amd import nrc as vn
+    w = vn.init_weights(vn.NRCConfig(seed=1, encoding=vn.ENCODING_SINCOS6)).astype(np.float32)
+    n = 16384
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        o.infer(w, 5, queries[reps * n % (queries.shape[0] - n):][:n], 1, clamp0=True)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s * 0.6:
+            break
+    t_inf = (time.perf_counter() - t0) / reps
+    t1 = time.perf_counter()
+    treps = 0
+    while True:
+        o.gradient(w, 5, records[:BATCH], 1, 0)
+        treps += 1
+        if time.perf_counter() - t1 > budget_s * 0.4:
+            break
+    t_tr = (time.perf_counter() - t1) / treps
+    return {"value": n / t_inf, "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} x {n} SINCOS6 queries inference (FP32, oracle/nrc_oracle.c REAL=float, OpenMP)",
+            "train_samples_per_s": BATCH / t_tr,
+            "train_sample": f"{treps} x {BATCH}-record forward+backward (no optimizer)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--plain-reps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the NRC hot path is measured per GPU "
+              "(caches shard across GPUs with no communication); use --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    import vknrc_amd as v
+    from vknrc_amd import workloads as wl
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    cfg = v.NRCConfig(seed=wl.config_seed(1), encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE,
+                      loss_scale=128.0, dynamic_loss_scale=True)
+    st = v.NRCState(cfg)
+    rng = np.random.default_rng(wl.config_seed(1, 1))
+    log("generating synthetic data ...")
+    q = wl.make_queries(QUERIES, rng)
+    lights = wl.GaussianLights(wl.config_seed(1, 2))
+    pool_frames = 8
+    recs = wl.make_train_records(pool_frames * BATCHES * BATCH, rng, lights)
+    dq = torch.from_numpy(q).to(dev)
+    drec = torch.from_numpy(recs).to(dev)
+    out = torch.zeros((QUERIES, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    def frame(i):
+        b = drec[(i % pool_frames) * BATCHES * BATCH:][:BATCHES * BATCH]
+        st.frame(dq, out, b, max_batch_records=BATCH, path=v.PATH_FUSED_F16, stream=stream)
+
+    log("warmup ...")
+    for i in range(args.warmup):
+        frame(i)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    log(f"timing {K} frames ...")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        b = drec[(i % pool_frames) * BATCHES * BATCH:][:BATCHES * BATCH]
+        ev[i][0].record(stream)
+        st.infer(dq, out, path=v.PATH_FUSED_F16, stream=stream)
+        ev[i][1].record(stream)
+        for j in range(BATCHES):
+            st.train(b[j * BATCH:(j + 1) * BATCH], path=v.PATH_FUSED_F16, write_use_weights=(j == BATCHES - 1),
+                     stream=stream)
+        ev[i][2].record(stream)
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - t0
+    infer_ms = np.array([ev[i][0].elapsed_time(ev[i][1]) for i in range(K)])
+    train_ms = np.array([ev[i][1].elapsed_time(ev[i][2]) for i in range(K)])
+    frame_ms = infer_ms + train_ms
+    s = st.get_state()["state"]
+    log(f"loss {s['last_loss']:.4f} scale {s['loss_scale']} t {s['t']} skipped {s['skipped_steps']}")
+
+    # plain FP32 inference at the same size (the plain method)
+    plain_ms = []
+    for _ in range(args.plain_reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        st.infer(dq, out, path=v.PATH_PLAIN_F32, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms.append(e0.elapsed_time(e1))
+    plain_ms = float(np.mean(plain_ms[1:]))
+
+    infer_avg_s = float(infer_ms.mean()) / 1e3
+    achieved = INFER_FLOP_PER_QUERY * QUERIES / infer_avg_s / 1e12
+    res = {
+        "metric": "fused-MLP infer queries/s + train samples/s (64-wide); p99 frame latency",
+        "value": K * QUERIES / total_s,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": total_s * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16-mfma/fp32-acc",
+        "data": "synthetic (seeded Gaussian-light radiance targets, random-init Kaiming weights)",
+        "config": {"workload": "single cache, 2,073,600 queries + 4 x 16,384 training records per frame, "
+                               "64->5x64->3, SINCOS6 encoding, relative-L2, dynamic loss scale 128",
+                   "queries_per_frame": QUERIES, "train_batches": BATCHES, "train_batch": BATCH},
+        "infer_queries_per_s": QUERIES / infer_avg_s,
+        "train_samples_per_s": BATCHES * BATCH / (float(train_ms.mean()) / 1e3),
+        "frame_ms_p50": float(np.percentile(frame_ms, 50)),
+        "frame_ms_p99": float(np.percentile(frame_ms, 99)),
+        "infer_ms_avg": float(infer_ms.mean()),
+        "train_step_ms_avg": float(train_ms.mean()) / BATCHES,
+        "plain_f32_infer_queries_per_s": QUERIES / (plain_ms / 1e3),
+        "fused_vs_plain_infer": (QUERIES / infer_avg_s) / (QUERIES / (plain_ms / 1e3)),
+        "roofline": {"bound": "mfma", "kernel": "fused_infer_kernel", "achieved": achieved,
+                     "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_F16_TFLOPS,
+                     "traffic": None,
+                     "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(q, recs, args.cpu_seconds)
+    print(json.dumps(res), flush=True)
+    st.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

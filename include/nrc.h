@@ -1,0 +1,200 @@
+/*
+ * nrc.h -- C ABI of the MI355X neural-radiance-cache hot path (libvknrc.so).
+ *
+ * Drop-in boundary for VkNRC's NRC state object and NN passes: a cache owns the
+ * network weights and optimizer state (VkNRCState), nrc_infer replaces the
+ * NNDispatch<NNInference> pass, nrc_train replaces the NNTrain pass group
+ * (zero gradient -> prepare -> gradient -> optimize).  Plain pointers, sizes and
+ * status codes only; device pointers are HIP device memory, streams are
+ * hipStream_t passed as void* (NULL = the null stream).  Calls on one cache are
+ * ordered by the stream they are issued on; distinct caches may be driven from
+ * distinct host threads.  No call aborts the process.
+ */
+#ifndef NRC_H
+#define NRC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Network shape: 64 -> hidden x (64x64, ReLU) -> 3, no bias.
+ * src/VkNRCState.hpp:23-25 (kNNHiddenLayers = 5, kNNWidth = 64, kNNOutWidth = 3). */
+#define NRC_WIDTH 64
+#define NRC_OUTPUT_WIDTH 3
+#define NRC_MAX_HIDDEN_LAYERS 5
+/* Constant.glsl:6-7: 4 train batches of at most 16384 records per frame. */
+#define NRC_TRAIN_BATCH_COUNT 4
+#define NRC_DEFAULT_TRAIN_BATCH_SIZE 16384
+/* Largest train batch one nrc_train call accepts. */
+#define NRC_MAX_TRAIN_BATCH_SIZE 65536
+
+typedef enum nrc_status {
+	NRC_OK = 0,
+	NRC_ERROR_INVALID_ARGUMENT = 1,
+	NRC_ERROR_UNSUPPORTED = 2, /* NNInferenceShader.cpp:33-34 returns a null pipeline in this case */
+	NRC_ERROR_OUT_OF_MEMORY = 3,
+	NRC_ERROR_HIP = 4,
+	NRC_ERROR_CAPACITY = 5
+} nrc_status;
+
+/* Kernel form.  PLAIN_F32 is the plain method: FP32 features, weights and
+ * accumulation.  FUSED_F16 runs the layer GEMMs on FP16 MFMA with FP32
+ * accumulation, FP16 activations and FP32 master weights. */
+typedef enum nrc_path { NRC_PATH_PLAIN_F32 = 0, NRC_PATH_FUSED_F16 = 1 } nrc_path;
+
+/* Input encoding (NRCRecord.glsl:78-95).  TRI12 is the reference layout.
+ * SINCOS6 replaces each position axis's 12 triangle waves by sin(2^k pi p),
+ * k = 0..5, followed by cos(2^k pi p).  UV_SINCOS16 reads (u, v) from
+ * position[0..1] and emits sin/cos(2^k pi u), k = 0..15, then the same for v. */
+typedef enum nrc_encoding { NRC_ENCODING_TRI12 = 0, NRC_ENCODING_SINCOS6 = 1, NRC_ENCODING_UV_SINCOS16 = 2 } nrc_encoding;
+
+/* NNLoadDA3_RelativeL2LuminanceLoss / NNLoadDA3_L2Loss, NN_nv.glsl:162-196. */
+typedef enum nrc_loss { NRC_LOSS_RELATIVE_L2_LUMINANCE = 0, NRC_LOSS_L2 = 1 } nrc_loss;
+
+/* VkNRCState.cpp:39-44 is Kaiming normal N(0, sqrt(2/64)) for every weight. */
+typedef enum nrc_init { NRC_INIT_KAIMING_NORMAL = 0, NRC_INIT_XAVIER_UNIFORM = 1 } nrc_init;
+
+/* One query: UnpackedNRCInput, NRCRecord.glsl:40-45 (14 floats, 56 bytes).
+ * scattered_dir and normal are already sphere-encoded to [0,1]^2 (NRCSphEncode). */
+typedef struct nrc_query {
+	float position[3];
+	float scattered_dir[2];
+	float normal[2];
+	float roughness;
+	float diffuse[3];
+	float specular[3];
+} nrc_query;
+
+/* One training record: the query plus its target radiance (68 bytes).  The
+ * reference's NRCTrainRecord (NRCRecord.glsl:35-38) carries the target in its
+ * bias field once the inference epilogue has back-filled it. */
+typedef struct nrc_train_record {
+	nrc_query query;
+	float target[3];
+} nrc_train_record;
+
+/* OptimizerEntry, VkNRCState.cpp:29-31 / nrc_optimize.comp:19-22. */
+typedef struct nrc_optimizer_entry {
+	float m, v, weight, ema_weight;
+} nrc_optimizer_entry;
+
+/* OptimizerState, VkNRCState.cpp:25-28 / nrc_train_prepare.comp:11-14, extended
+ * with the dynamic loss-scale state and the last step's loss. */
+typedef struct nrc_optimizer_state {
+	uint32_t t;
+	float beta1_t, beta2_t, alpha_t, alpha_t_1;
+	float loss_scale;
+	uint32_t clean_steps;   /* steps since the loss scale last changed */
+	uint32_t skipped_steps; /* steps skipped because the gradient overflowed */
+	float last_loss;        /* mean per-record loss of the last non-empty step */
+	uint32_t last_count;    /* records used by the last non-empty step */
+} nrc_optimizer_state;
+
+typedef struct nrc_cache_desc {
+	uint32_t hidden_layers; /* 5 (reference) or 4 */
+	nrc_encoding encoding;
+	nrc_loss loss;
+	nrc_init init;
+	uint64_t seed;                 /* weight init seed (the reference draws std::random_device) */
+	uint32_t train_batch_size;     /* per-step clamp, nrc_train_prepare.comp:17-19 (16384) */
+	float learning_rate;           /* nrc_optimize.comp:10 (0.002) */
+	float beta1, beta2, epsilon;   /* Constant.glsl:11-12, nrc_optimize.comp:11 (0.9, 0.999, 1e-8) */
+	float ema_alpha;               /* Constant.glsl:13 (0.99) */
+	float loss_scale;              /* Constant.glsl:8 LOSS_SCALE (1.0) or the dynamic start value */
+	int32_t dynamic_loss_scale;    /* 1: skip the step and halve the scale on overflow */
+	uint32_t loss_scale_growth_interval; /* clean steps before the scale doubles */
+	int32_t use_ema_weights;       /* VkNRCState::SetUseEMAWeights, default off */
+} nrc_cache_desc;
+
+typedef struct nrc_cache nrc_cache;
+
+/* Flags of nrc_train. */
+#define NRC_TRAIN_WRITE_USE_WEIGHTS 1u /* -DWRITE_USE_WEIGHTS variant of nrc_optimize.comp:46-53 */
+
+const char *nrc_status_string(nrc_status status);
+/* Message of the last failed call on this host thread ("" when none). */
+const char *nrc_last_error(void);
+
+/* Fills the reference constants: 5 hidden layers, TRI12, relative L2,
+ * Kaiming init, batch 16384, Adam 0.002/0.9/0.999/1e-8, EMA 0.99, scale 1. */
+void nrc_cache_desc_default(nrc_cache_desc *desc);
+
+/* Weight count: hidden * 64 * 64 + 3 * 64 (VkNRCState::GetWeightCount, VkNRCState.hpp:25,88);
+ * 0 for an unsupported depth. */
+uint32_t nrc_weight_count(uint32_t hidden_layers);
+
+/* Host-only: the FP32 initial weights a cache created from desc with this seed
+ * starts from (VkNRCState::initialize_weights, VkNRCState.cpp:39-44). */
+nrc_status nrc_init_weights(const nrc_cache_desc *desc, uint64_t seed, float *host_weights);
+
+/* VkNRCState ctor + ResetMLPBuffers (VkNRCState.hpp:42-45, VkNRCState.cpp:46-88):
+ * allocates the cache on the current HIP device and initialises it (synchronous). */
+nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out_cache);
+/* ResetMLPBuffers ("Re-Train", src/main.cpp:144-147) with a new seed (synchronous). */
+nrc_status nrc_cache_reset(nrc_cache *cache, uint64_t seed);
+void nrc_cache_destroy(nrc_cache *cache);
+nrc_status nrc_cache_get_desc(const nrc_cache *cache, nrc_cache_desc *desc);
+/* SetUseEMAWeights, VkNRCState.hpp:69. Takes effect at the next use-weights write. */
+nrc_status nrc_set_use_ema(nrc_cache *cache, int32_t use_ema);
+
+/*
+ * NNDispatch<NNInference> (NNDispatch.hpp:13-57, NNInference.cpp:11-71,
+ * nrc_inference.comp:30-48): rgb[i] = max(MLP(encode(queries[i])), 0) for
+ * i < min(*d_count, max_count), using the cache's use-weights.  d_count is a
+ * device pointer (NULL: count = max_count); d_rgb is [max_count][3] floats.
+ */
+nrc_status nrc_infer(nrc_cache *cache, const nrc_query *d_queries, const uint32_t *d_count, uint32_t max_count,
+                     float *d_rgb, nrc_path path, void *stream);
+
+/*
+ * NNTrain (NNTrain.hpp:17-128, NNTrain.cpp:11-133): one optimizer step on the
+ * first min(*d_count, max_count, train_batch_size) records.  An empty batch
+ * changes nothing (nrc_optimize.comp:33-34).  max_count <= NRC_MAX_TRAIN_BATCH_SIZE.
+ */
+nrc_status nrc_train(nrc_cache *cache, const nrc_train_record *d_records, const uint32_t *d_count,
+                     uint32_t max_count, nrc_path path, uint32_t flags, void *stream);
+
+/*
+ * One frame of NRCRenderGraph (NRCRenderGraph.cpp:46-80): inference with the
+ * use-weights, then NRC_TRAIN_BATCH_COUNT chained train steps, the last of
+ * which writes the use-weights.  d_batch_counts holds NRC_TRAIN_BATCH_COUNT
+ * device counts (NULL: every batch uses max_batch_records); d_batches holds
+ * NRC_TRAIN_BATCH_COUNT * max_batch_records records.
+ */
+nrc_status nrc_frame(nrc_cache *cache, const nrc_query *d_queries, const uint32_t *d_query_count,
+                     uint32_t max_queries, float *d_rgb, const nrc_train_record *d_batches,
+                     const uint32_t *d_batch_counts, uint32_t max_batch_records, nrc_path path, void *stream);
+
+/* Synchronous state export/import (the buffers GetWeightBuffer,
+ * GetUseWeightBuffer, GetOptimizerEntryBuffer, GetOptimizerStateBuffer expose,
+ * VkNRCState.hpp:49-52).  Any pointer may be NULL to skip that part.  fp16
+ * arrays are IEEE binary16 bit patterns.  nrc_set_state re-derives the fp16
+ * weights (and the use-weights, as weight or EMA per use_ema) from the entries. */
+nrc_status nrc_get_state(nrc_cache *cache, nrc_optimizer_entry *host_entries, nrc_optimizer_state *host_state,
+                         uint16_t *host_weights_f16, uint16_t *host_use_weights_f16);
+nrc_status nrc_set_state(nrc_cache *cache, const nrc_optimizer_entry *host_entries,
+                         const nrc_optimizer_state *host_state);
+
+/* NRCInputEncode (NRCRecord.glsl:78-95) alone: d_features is [n][64] FP32
+ * (what the plain path feeds its first layer). */
+nrc_status nrc_encode(nrc_encoding encoding, const nrc_query *d_queries, uint32_t n, float *d_features, void *stream);
+/* NRCSphEncode (NRCRecord.glsl:47-49): d_dirs [n][3] -> d_uv [n][2]. */
+nrc_status nrc_sph_encode(const float *d_dirs, uint32_t n, float *d_uv, void *stream);
+
+/* The reference test kernels over pre-encoded inputs (test/evaluate_NV.comp:16-31,
+ * test/train_NV.comp:18-46): fp16 weights [weight_count], fp16 inputs [n][64].
+ * forward: d_out [n][3] FP32 raw outputs.  gradient: L2 loss against d_targets
+ * [n][3] FP32, d_dw [weight_count] FP32 summed over the n samples (overwritten). */
+nrc_status nrc_mlp_forward_f16(uint32_t hidden_layers, const uint16_t *d_weights_f16, const uint16_t *d_inputs_f16,
+                               uint32_t n, float *d_out, void *stream);
+nrc_status nrc_mlp_gradient_f16(uint32_t hidden_layers, const uint16_t *d_weights_f16, const uint16_t *d_inputs_f16,
+                                const float *d_targets, uint32_t n, float *d_dw, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NRC_H */

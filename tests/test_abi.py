@@ -1,0 +1,109 @@
+"""CPU checks of the C-ABI library: it loads, exports every function include/nrc.h
+declares, and its host-only entry points behave (no GPU calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import vknrc_amd as v
+from vknrc_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "nrc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(nrc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for f in ("nrc_cache_create", "nrc_cache_reset", "nrc_cache_destroy", "nrc_infer", "nrc_train", "nrc_frame",
+              "nrc_get_state", "nrc_set_state", "nrc_set_use_ema", "nrc_encode", "nrc_mlp_forward_f16",
+              "nrc_mlp_gradient_f16"):
+        assert f in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(v.nrc.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_defaults_are_the_reference_constants():
+    d = v.default_config()
+    assert d.hidden_layers == 5 and d.train_batch_size == 16384           # VkNRCState.hpp:23
+    assert abs(d.learning_rate - 0.002) < 1e-9 and abs(d.epsilon - 1e-8) < 1e-15  # nrc_optimize.comp:10-11
+    assert abs(d.beta1 - 0.9) < 1e-7 and abs(d.beta2 - 0.999) < 1e-7 and abs(d.ema_alpha - 0.99) < 1e-7
+    assert d.loss_scale == 1.0 and not d.dynamic_loss_scale and not d.use_ema_weights
+    assert d.encoding == v.ENCODING_TRI12 and d.loss == v.LOSS_RELATIVE_L2_LUMINANCE
+
+
+def test_weight_count():
+    lib = v.load_library()
+    assert lib.nrc_weight_count(5) == 20672 == v.weight_count(5)   # VkNRCState.hpp:25
+    assert lib.nrc_weight_count(4) == 16576
+    assert lib.nrc_weight_count(3) == 0 and lib.nrc_weight_count(6) == 0
+
+
+def test_init_weights_seeded_kaiming():
+    a = v.init_weights(v.NRCConfig(seed=7))
+    b = v.init_weights(v.NRCConfig(seed=7))
+    c = v.init_weights(v.NRCConfig(seed=8))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.size == 20672
+    assert abs(a.std() - np.sqrt(2 / 64)) < 0.005 and abs(a.mean()) < 0.005    # VkNRCState.cpp:41
+
+
+def test_init_weights_xavier_uniform_bounds():
+    w = v.init_weights(v.NRCConfig(seed=3, hidden_layers=4, init=v.INIT_XAVIER_UNIFORM))
+    hid, out = w[:4 * 4096], w[4 * 4096:]
+    assert np.abs(hid).max() <= np.sqrt(6 / 128) and np.abs(hid).max() > 0.95 * np.sqrt(6 / 128)
+    assert np.abs(out).max() <= np.sqrt(6 / 67)
+
+
+@pytest.mark.parametrize("field,value", [("hidden_layers", 3), ("encoding", 7), ("loss", 9), ("train_batch_size", 0),
+                                         ("train_batch_size", 70000), ("learning_rate", -1.0), ("beta2", 1.0),
+                                         ("loss_scale", 0.0)])
+def test_invalid_desc_is_rejected_with_a_message(field, value):
+    cfg = v.NRCConfig()
+    setattr(cfg, field, value)
+    with pytest.raises(v.NRCError) as e:
+        v.init_weights(cfg)
+    assert e.value.status == 1 and str(e.value)
+
+
+def test_null_arguments_return_errors_not_aborts():
+    lib = v.load_library()
+    assert lib.nrc_infer(None, None, None, 10, None, 0, None) == 1
+    assert lib.nrc_train(None, None, None, 10, 0, 0, None) == 1
+    assert lib.nrc_infer(None, None, None, 0, None, 0, None) == 1       # NULL cache is checked first
+    assert lib.nrc_encode(5, None, 3, None, None) == 1
+    assert lib.nrc_mlp_forward_f16(3, None, None, 1, None, None) == 1
+    assert lib.nrc_last_error().decode()
+    assert lib.nrc_status_string(4).decode() == "HIP error"
+
+
+def test_struct_layouts_match_the_header():
+    assert C.sizeof(v.nrc.OptimizerState) == 40
+    assert C.sizeof(v.nrc.CacheDesc) == 64
+
+
+def test_workload_generators():
+    rng = np.random.default_rng(0)
+    q = wl.make_queries(1000, rng)
+    assert q.dtype == np.float32 and q.shape == (1000, 14)
+    assert np.all(np.abs(q[:, 0:3]) <= 1) and np.all((q[:, 3:7] >= 0) & (q[:, 3:7] <= 1))
+    lights = wl.GaussianLights(5)
+    r = wl.make_train_records(2000, rng, lights)
+    assert r.shape == (2000, 17) and np.all(r[:, 14:] >= 0) and np.isfinite(r).all()
+    img = wl.make_image(1)
+    assert img.shape == (256, 256, 3) and img.min() >= 0 and img.max() <= 1
+    ir = wl.make_image_records(100, rng, img)
+    # bilinear sampling at texel centres returns the texel
+    uv = np.array([[0.5 / 256, 0.5 / 256], [100.5 / 256, 7.5 / 256]])
+    assert np.allclose(wl.sample_bilinear(img, uv), img[[0, 7], [0, 100]], atol=1e-6)
+    assert ir.shape == (100, 17)

@@ -1,0 +1,438 @@
+// nrc_api.cpp -- the C ABI of include/nrc.h: the cache object (VkNRCState) and
+// the NN passes (NNInference, NNTrain) as stream-ordered HIP launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nrc_internal.hpp"
+
+using nrc::DeviceState;
+
+struct nrc_cache {
+	nrc_cache_desc desc;
+	int device = 0;
+	uint32_t wc = 0;
+	nrc_optimizer_entry *entries = nullptr;
+	_Float16 *w16 = nullptr, *use16 = nullptr;
+	float *w32 = nullptr, *use32 = nullptr;
+	DeviceState *state = nullptr;
+	float *partials = nullptr;
+	uint32_t part_cap = 0, pstride = 0;
+	// plain-path scratch (allocated on the first plain train step)
+	float *acts = nullptr, *deltas = nullptr, *sample_loss = nullptr;
+	uint32_t ldn = 0;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+nrc_status fail(nrc_status s, const std::string &msg) {
+	g_error = msg;
+	return s;
+}
+
+nrc_status hip_fail(hipError_t e, const char *what) {
+	return fail(e == hipErrorOutOfMemory ? NRC_ERROR_OUT_OF_MEMORY : NRC_ERROR_HIP,
+	            std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NRC_HIP(call, what)                                                                                            \
+	do {                                                                                                               \
+		hipError_t e_ = (call);                                                                                        \
+		if (e_ != hipSuccess) return hip_fail(e_, what);                                                               \
+	} while (0)
+
+// SplitMix64 stream; uniform doubles in [0, 1) from the top 53 bits.
+struct SplitMix64 {
+	uint64_t s;
+	uint64_t next() {
+		uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+		return z ^ (z >> 31);
+	}
+	double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+	double normal() { // Box-Muller
+		const double u1 = 1.0 - uniform(), u2 = uniform();
+		return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586476925 * u2);
+	}
+};
+
+bool valid_desc(const nrc_cache_desc *d, std::string &why) {
+	if (!d) return why = "desc is NULL", false;
+	if (d->hidden_layers != 4 && d->hidden_layers != 5) return why = "hidden_layers must be 4 or 5", false;
+	if ((unsigned)d->encoding > NRC_ENCODING_UV_SINCOS16) return why = "unknown encoding", false;
+	if ((unsigned)d->loss > NRC_LOSS_L2) return why = "unknown loss", false;
+	if ((unsigned)d->init > NRC_INIT_XAVIER_UNIFORM) return why = "unknown init", false;
+	if (d->train_batch_size == 0 || d->train_batch_size > NRC_MAX_TRAIN_BATCH_SIZE)
+		return why = "train_batch_size must be in [1, NRC_MAX_TRAIN_BATCH_SIZE]", false;
+	if (!(d->learning_rate > 0) || !(d->beta1 >= 0 && d->beta1 < 1) || !(d->beta2 >= 0 && d->beta2 < 1) ||
+	    !(d->epsilon >= 0) || !(d->ema_alpha >= 0 && d->ema_alpha < 1))
+		return why = "optimizer hyper-parameters out of range", false;
+	if (!(d->loss_scale > 0) || !std::isfinite(d->loss_scale)) return why = "loss_scale must be positive", false;
+	return true;
+}
+
+void host_init_weights(const nrc_cache_desc *d, uint64_t seed, float *w) {
+	SplitMix64 rng{seed};
+	const uint32_t H = d->hidden_layers, wc = nrc_weight_count(H);
+	if (d->init == NRC_INIT_KAIMING_NORMAL) {
+		// VkNRCState.cpp:39-44: N(0, sqrt(2 / 64)) for every weight, output layer included
+		const double sigma = std::sqrt(2.0 / 64.0);
+		for (uint32_t i = 0; i < wc; ++i) w[i] = (float)(sigma * rng.normal());
+	} else {
+		// Glorot uniform: U(-a, a), a = sqrt(6 / (fan_in + fan_out))
+		for (uint32_t i = 0; i < wc; ++i) {
+			const double fan_out = i < H * 4096u ? 64.0 : 3.0;
+			const double a = std::sqrt(6.0 / (64.0 + fan_out));
+			w[i] = (float)((2.0 * rng.uniform() - 1.0) * a);
+		}
+	}
+}
+
+// Uploads entries + the derived fp16/fp32 weight copies (ResetMLPBuffers, VkNRCState.cpp:46-88).
+nrc_status upload_entries(nrc_cache *c, const nrc_optimizer_entry *entries) {
+	std::vector<_Float16> w16(c->wc), u16(c->wc);
+	std::vector<float> w32(c->wc), u32(c->wc);
+	for (uint32_t i = 0; i < c->wc; ++i) {
+		w32[i] = entries[i].weight;
+		u32[i] = c->desc.use_ema_weights ? entries[i].ema_weight : entries[i].weight;
+		w16[i] = (_Float16)w32[i];
+		u16[i] = (_Float16)u32[i];
+	}
+	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
+	NRC_HIP(hipMemcpy(c->entries, entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyHostToDevice), "upload entries");
+	NRC_HIP(hipMemcpy(c->w16, w16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->use16, u16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload use weights");
+	NRC_HIP(hipMemcpy(c->w32, w32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->use32, u32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload use weights");
+	return NRC_OK;
+}
+
+nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
+	std::vector<float> w(c->wc);
+	host_init_weights(&c->desc, seed, w.data());
+	std::vector<nrc_optimizer_entry> e(c->wc);
+	for (uint32_t i = 0; i < c->wc; ++i) e[i] = nrc_optimizer_entry{0.0f, 0.0f, w[i], w[i]};
+	nrc_status s = upload_entries(c, e.data());
+	if (s != NRC_OK) return s;
+	NRC_HIP(nrc::launch_init_state(c->state, c->desc.loss_scale, nullptr), "init state");
+	NRC_HIP(hipDeviceSynchronize(), "reset");
+	c->desc.seed = seed;
+	return NRC_OK;
+}
+
+void free_cache(nrc_cache *c) {
+	if (!c) return;
+	(void)hipSetDevice(c->device);
+	void *ptrs[] = {c->entries, c->w16, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
+	for (void *p : ptrs)
+		if (p) (void)hipFree(p);
+	delete c;
+}
+
+nrc::AdamParams adam_params(const nrc_cache *c) {
+	nrc::AdamParams p;
+	p.lr = c->desc.learning_rate;
+	p.beta1 = c->desc.beta1;
+	p.beta2 = c->desc.beta2;
+	p.eps = c->desc.epsilon;
+	p.ema_alpha = c->desc.ema_alpha;
+	p.dynamic_scale = c->desc.dynamic_loss_scale;
+	p.growth_interval = c->desc.loss_scale_growth_interval;
+	p.use_ema = c->desc.use_ema_weights;
+	return p;
+}
+
+nrc_status ensure_plain_scratch(nrc_cache *c) {
+	if (c->acts) return NRC_OK;
+	const uint32_t ldn = (c->desc.train_batch_size + 63u) & ~63u;
+	const size_t slab = (size_t)(c->desc.hidden_layers + 1) * 64 * ldn;
+	NRC_HIP(hipMalloc(&c->acts, slab * sizeof(float)), "plain scratch");
+	NRC_HIP(hipMalloc(&c->deltas, slab * sizeof(float)), "plain scratch");
+	NRC_HIP(hipMalloc(&c->sample_loss, (size_t)ldn * sizeof(float)), "plain scratch");
+	c->ldn = ldn;
+	return NRC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *nrc_status_string(nrc_status s) {
+	switch (s) {
+	case NRC_OK: return "ok";
+	case NRC_ERROR_INVALID_ARGUMENT: return "invalid argument";
+	case NRC_ERROR_UNSUPPORTED: return "unsupported";
+	case NRC_ERROR_OUT_OF_MEMORY: return "out of memory";
+	case NRC_ERROR_HIP: return "HIP error";
+	case NRC_ERROR_CAPACITY: return "capacity exceeded";
+	}
+	return "unknown status";
+}
+
+const char *nrc_last_error(void) { return g_error.c_str(); }
+
+void nrc_cache_desc_default(nrc_cache_desc *d) {
+	if (!d) return;
+	std::memset(d, 0, sizeof(*d));
+	d->hidden_layers = 5;
+	d->encoding = NRC_ENCODING_TRI12;
+	d->loss = NRC_LOSS_RELATIVE_L2_LUMINANCE;
+	d->init = NRC_INIT_KAIMING_NORMAL;
+	d->seed = 0;
+	d->train_batch_size = NRC_DEFAULT_TRAIN_BATCH_SIZE;
+	d->learning_rate = 0.002f;
+	d->beta1 = 0.9f;
+	d->beta2 = 0.999f;
+	d->epsilon = 1e-8f;
+	d->ema_alpha = 0.99f;
+	d->loss_scale = 1.0f;
+	d->dynamic_loss_scale = 0;
+	d->loss_scale_growth_interval = 2000;
+	d->use_ema_weights = 0;
+}
+
+uint32_t nrc_weight_count(uint32_t hidden_layers) {
+	return (hidden_layers == 4 || hidden_layers == 5) ? (uint32_t)nrc::weight_count((int)hidden_layers) : 0u;
+}
+
+nrc_status nrc_init_weights(const nrc_cache_desc *desc, uint64_t seed, float *host_weights) {
+	std::string why;
+	if (!valid_desc(desc, why)) return fail(NRC_ERROR_INVALID_ARGUMENT, why);
+	if (!host_weights) return fail(NRC_ERROR_INVALID_ARGUMENT, "host_weights is NULL");
+	host_init_weights(desc, seed, host_weights);
+	return NRC_OK;
+}
+
+nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
+	std::string why;
+	if (!out) return fail(NRC_ERROR_INVALID_ARGUMENT, "out_cache is NULL");
+	*out = nullptr;
+	if (!valid_desc(desc, why)) return fail(NRC_ERROR_INVALID_ARGUMENT, why);
+	nrc_cache *c = new nrc_cache();
+	c->desc = *desc;
+	c->wc = nrc_weight_count(desc->hidden_layers);
+	c->pstride = ((c->wc + 1) + 63u) & ~63u;
+	c->part_cap = (desc->train_batch_size + nrc::kFusedTrainSamples - 1) / nrc::kFusedTrainSamples;
+	hipError_t e = hipGetDevice(&c->device);
+	if (e == hipSuccess) e = hipMalloc(&c->entries, sizeof(nrc_optimizer_entry) * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->w16, 2 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->state, sizeof(DeviceState));
+	if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(float) * (size_t)c->pstride * c->part_cap);
+	if (e != hipSuccess) {
+		free_cache(c);
+		return hip_fail(e, "nrc_cache_create");
+	}
+	nrc_status s = reset_cache(c, desc->seed);
+	if (s != NRC_OK) {
+		free_cache(c);
+		return s;
+	}
+	*out = c;
+	return NRC_OK;
+}
+
+nrc_status nrc_cache_reset(nrc_cache *c, uint64_t seed) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	return reset_cache(c, seed);
+}
+
+void nrc_cache_destroy(nrc_cache *c) { free_cache(c); }
+
+nrc_status nrc_cache_get_desc(const nrc_cache *c, nrc_cache_desc *d) {
+	if (!c || !d) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL argument");
+	*d = c->desc;
+	return NRC_OK;
+}
+
+nrc_status nrc_set_use_ema(nrc_cache *c, int32_t use_ema) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	c->desc.use_ema_weights = use_ema ? 1 : 0;
+	return NRC_OK;
+}
+
+nrc_status nrc_infer(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d_count, uint32_t max_count,
+                     float *d_rgb, nrc_path path, void *stream) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	if (max_count == 0) return NRC_OK;
+	if (!d_queries || !d_rgb) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	const int H = (int)c->desc.hidden_layers, enc = (int)c->desc.encoding;
+	const float *q = reinterpret_cast<const float *>(d_queries);
+	hipError_t e;
+	switch (path) {
+	case NRC_PATH_PLAIN_F32: e = nrc::launch_plain_infer(H, enc, q, d_count, max_count, c->use32, d_rgb, s); break;
+	case NRC_PATH_FUSED_F16: e = nrc::launch_fused_infer(H, enc, q, d_count, max_count, c->use16, d_rgb, s); break;
+	default: return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	}
+	if (e != hipSuccess) return hip_fail(e, "nrc_infer launch");
+	return NRC_OK;
+}
+
+nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint32_t *d_count, uint32_t max_count,
+                     nrc_path path, uint32_t flags, void *stream) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	if (max_count == 0) return NRC_OK;
+	if (!d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL records");
+	if (max_count > NRC_MAX_TRAIN_BATCH_SIZE) return fail(NRC_ERROR_CAPACITY, "max_count > NRC_MAX_TRAIN_BATCH_SIZE");
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	nrc::TrainArgs a{};
+	a.hidden = (int)c->desc.hidden_layers;
+	a.encoding = (int)c->desc.encoding;
+	a.loss = (int)c->desc.loss;
+	a.records = reinterpret_cast<const float *>(d_records);
+	a.d_count = d_count;
+	a.max_count = max_count;
+	a.clamp = c->desc.train_batch_size;
+	a.w32 = c->w32;
+	a.w16 = c->w16;
+	a.state = c->state;
+	a.partials = c->partials;
+	a.pstride = c->pstride;
+	// partial-gradient slots the launch will write, checked before anything runs
+	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
+	const uint32_t per = path == NRC_PATH_PLAIN_F32 ? nrc::kPlainChunk : nrc::kFusedTrainSamples;
+	const uint32_t need = (n + per - 1) / per;
+	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
+	uint32_t nparts = 0;
+	hipError_t e;
+	if (path == NRC_PATH_PLAIN_F32) {
+		nrc_status st = ensure_plain_scratch(c);
+		if (st != NRC_OK) return st;
+		a.acts = c->acts;
+		a.deltas = c->deltas;
+		a.sample_loss = c->sample_loss;
+		a.ldn = c->ldn;
+		e = nrc::launch_plain_gradient(a, &nparts, s);
+	} else {
+		e = nrc::launch_fused_gradient(a, &nparts, s);
+	}
+	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
+	nrc::AdamArgs o{};
+	o.partials = c->partials;
+	o.nparts = nparts;
+	o.pstride = c->pstride;
+	o.d_count = d_count;
+	o.max_count = max_count;
+	o.clamp = c->desc.train_batch_size;
+	o.state = c->state;
+	o.entries = c->entries;
+	o.w16 = c->w16;
+	o.use16 = c->use16;
+	o.w32 = c->w32;
+	o.use32 = c->use32;
+	o.write_use = (flags & NRC_TRAIN_WRITE_USE_WEIGHTS) ? 1 : 0;
+	o.weight_count = c->wc;
+	o.p = adam_params(c);
+	e = nrc::launch_adam(o, s);
+	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
+	return NRC_OK;
+}
+
+nrc_status nrc_frame(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d_query_count, uint32_t max_queries,
+                     float *d_rgb, const nrc_train_record *d_batches, const uint32_t *d_batch_counts,
+                     uint32_t max_batch_records, nrc_path path, void *stream) {
+	nrc_status st = nrc_infer(c, d_queries, d_query_count, max_queries, d_rgb, path, stream);
+	if (st != NRC_OK) return st;
+	for (uint32_t b = 0; b < NRC_TRAIN_BATCH_COUNT; ++b) {
+		const uint32_t flags = (b == NRC_TRAIN_BATCH_COUNT - 1) ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u;
+		st = nrc_train(c, d_batches ? d_batches + (size_t)b * max_batch_records : nullptr,
+		               d_batch_counts ? d_batch_counts + b : nullptr, max_batch_records, path, flags, stream);
+		if (st != NRC_OK) return st;
+	}
+	return NRC_OK;
+}
+
+nrc_status nrc_get_state(nrc_cache *c, nrc_optimizer_entry *entries, nrc_optimizer_state *state, uint16_t *w16,
+                         uint16_t *use16) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
+	NRC_HIP(hipDeviceSynchronize(), "nrc_get_state");
+	if (entries)
+		NRC_HIP(hipMemcpy(entries, c->entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyDeviceToHost), "get entries");
+	if (state) NRC_HIP(hipMemcpy(state, &c->state->pub, sizeof(nrc_optimizer_state), hipMemcpyDeviceToHost), "get state");
+	if (w16) NRC_HIP(hipMemcpy(w16, c->w16, 2 * c->wc, hipMemcpyDeviceToHost), "get weights");
+	if (use16) NRC_HIP(hipMemcpy(use16, c->use16, 2 * c->wc, hipMemcpyDeviceToHost), "get use weights");
+	return NRC_OK;
+}
+
+nrc_status nrc_set_state(nrc_cache *c, const nrc_optimizer_entry *entries, const nrc_optimizer_state *state) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
+	NRC_HIP(hipDeviceSynchronize(), "nrc_set_state");
+	if (entries) {
+		nrc_status s = upload_entries(c, entries);
+		if (s != NRC_OK) return s;
+	}
+	if (state) {
+		DeviceState ds{};
+		ds.pub = *state;
+		NRC_HIP(hipMemcpy(c->state, &ds, sizeof(ds), hipMemcpyHostToDevice), "set state");
+	}
+	NRC_HIP(hipDeviceSynchronize(), "nrc_set_state");
+	return NRC_OK;
+}
+
+nrc_status nrc_encode(nrc_encoding encoding, const nrc_query *d_queries, uint32_t n, float *d_features, void *stream) {
+	if ((unsigned)encoding > NRC_ENCODING_UV_SINCOS16) return fail(NRC_ERROR_INVALID_ARGUMENT, "unknown encoding");
+	if (n == 0) return NRC_OK;
+	if (!d_queries || !d_features) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	hipError_t e = nrc::launch_encode((int)encoding, reinterpret_cast<const float *>(d_queries), n, d_features,
+	                                  reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_encode");
+	return NRC_OK;
+}
+
+nrc_status nrc_sph_encode(const float *d_dirs, uint32_t n, float *d_uv, void *stream) {
+	if (n == 0) return NRC_OK;
+	if (!d_dirs || !d_uv) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	hipError_t e = nrc::launch_sph_encode(d_dirs, n, d_uv, reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_sph_encode");
+	return NRC_OK;
+}
+
+nrc_status nrc_mlp_forward_f16(uint32_t hidden, const uint16_t *d_w, const uint16_t *d_in, uint32_t n, float *d_out,
+                               void *stream) {
+	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");
+	if (n == 0) return NRC_OK;
+	if (!d_w || !d_in || !d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	hipError_t e = nrc::launch_fused_forward_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), n,
+	                                             reinterpret_cast<const _Float16 *>(d_w), d_out,
+	                                             reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_forward_f16");
+	return NRC_OK;
+}
+
+nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint16_t *d_in, const float *d_targets,
+                                uint32_t n, float *d_dw, void *stream) {
+	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");
+	if (!d_w || !d_in || !d_targets || !d_dw) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (n > NRC_MAX_TRAIN_BATCH_SIZE) return fail(NRC_ERROR_CAPACITY, "n > NRC_MAX_TRAIN_BATCH_SIZE");
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	const uint32_t wc = nrc_weight_count(hidden);
+	const uint32_t pstride = ((wc + 1) + 63u) & ~63u;
+	const uint32_t nb = n == 0 ? 1 : (n + nrc::kFusedTrainSamples - 1) / nrc::kFusedTrainSamples;
+	float *partials = nullptr;
+	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s), "gradient scratch");
+	uint32_t nparts = 0;
+	hipError_t e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n,
+	                                              reinterpret_cast<const _Float16 *>(d_w), partials, pstride, &nparts, s);
+	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
+	hipError_t e2 = hipFreeAsync(partials, s);
+	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16");
+	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_gradient_f16 free");
+	return NRC_OK;
+}
+
+} // extern "C"

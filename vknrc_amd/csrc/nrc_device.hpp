@@ -1,0 +1,206 @@
+// nrc_device.hpp -- device-side definitions shared by the NRC kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nrc.h"
+
+namespace nrc {
+
+constexpr int kWidth = 64;
+constexpr int kOut = 3;
+constexpr int kQueryFloats = 14;
+constexpr int kTrainFloats = 17;
+
+__host__ __device__ constexpr int weight_count(int hidden) { return hidden * kWidth * kWidth + kOut * kWidth; }
+
+// Device-resident optimizer state: the public part plus the cross-block
+// bookkeeping of the optimizer kernel.
+struct DeviceState {
+	nrc_optimizer_state pub;
+	uint32_t done_blocks; // blocks of the running optimizer launch that finished
+	uint32_t overflow;    // set by gradient kernels when a partial is non-finite
+	uint32_t pad[2];
+};
+
+struct AdamParams {
+	float lr, beta1, beta2, eps, ema_alpha;
+	int32_t dynamic_scale;
+	uint32_t growth_interval;
+	int32_t use_ema;
+};
+
+__device__ __forceinline__ uint32_t effective_count(const uint32_t *d_count, uint32_t max_count, uint32_t clamp) {
+	uint32_t c = d_count ? *d_count : max_count;
+	c = c < max_count ? c : max_count;
+	return c < clamp ? c : clamp;
+}
+
+// ---------------------------------------------------------------- encoders
+// _nrc_tri, NRCRecord.glsl:65-68 with GLSL's floor-based mod(a, 2).
+__device__ __forceinline__ float tri(float x) {
+	float a = x - 0.5f;
+	float m = a - 2.0f * floorf(a * 0.5f);
+	return 2.0f * fabsf(m - 1.0f) - 1.0f;
+}
+
+// _quartic_cdf with inv_radius 4, NRCRecord.glsl:51-56.
+__device__ __forceinline__ float quartic_cdf(float d) {
+	float u = d * 4.0f;
+	float u2 = u * u;
+	float u4 = u2 * u2;
+	float c = (15.0f / 16.0f) * u * (1.0f - (2.0f / 3.0f) * u2 + (1.0f / 5.0f) * u4) + 0.5f;
+	return fminf(fmaxf(c, 0.0f), 1.0f);
+}
+
+// NRCOneBlob4Encode, NRCRecord.glsl:59-63.  Adjacent bins share a CDF value, so
+// five evaluations give the four bins.
+__device__ __forceinline__ void oneblob4(float x, float *o) {
+	float c0 = quartic_cdf(0.0f - x), c1 = quartic_cdf(0.25f - x), c2 = quartic_cdf(0.5f - x),
+	      c3 = quartic_cdf(0.75f - x), c4 = quartic_cdf(1.0f - x);
+	o[0] = c1 - c0;
+	o[1] = c2 - c1;
+	o[2] = c3 - c2;
+	o[3] = c4 - c3;
+}
+
+// sin/cos(2^k pi x) for k = 0..N-1.  Every fourth octave is evaluated directly
+// with sincospi at the exact argument 2^k x; the three between use the
+// double-angle recurrences sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a, whose
+// error at most doubles per step (<= 8x the direct error before re-anchoring).
+template <int N> __device__ __forceinline__ void sincos_octaves(float x, float *s, float *c) {
+	float sv = 0.0f, cv = 1.0f;
+#pragma unroll
+	for (int k = 0; k < N; ++k) {
+		if (k % 4 == 0) {
+			sincospif(x * (float)(1 << k), &sv, &cv);
+		} else {
+			float ns = 2.0f * sv * cv;
+			float nc = 1.0f - 2.0f * sv * sv;
+			sv = ns;
+			cv = nc;
+		}
+		s[k] = sv;
+		c[k] = cv;
+	}
+}
+
+// Features of feature block q (16 features: 16q .. 16q+15) of one record.
+// Layout (NRCInputEncode, NRCRecord.glsl:78-95), 64 features:
+//   0..35  position: 12 per axis (TRI12: tri(2^k p), k=0..11;
+//          SINCOS6: sin(2^k pi p), k=0..5, then cos(2^k pi p), k=0..5)
+//   36..39 oneblob4(scattered_dir.u)   40..43 oneblob4(scattered_dir.v)
+//   44..47 oneblob4(normal.u)          48..51 oneblob4(normal.v)
+//   52..55 oneblob4(1 - exp(-roughness))
+//   56..58 diffuse rgb   59..61 specular rgb   62, 63 = 1
+// UV_SINCOS16: sin(2^k pi u) k=0..15 | cos(2^k pi u) | sin(2^k pi v) | cos(2^k pi v).
+template <int ENC> __device__ __forceinline__ void axis12(float p, float *o) {
+	if constexpr (ENC == NRC_ENCODING_TRI12) {
+#pragma unroll
+		for (int k = 0; k < 12; ++k) o[k] = tri(p * (float)(1 << k));
+	} else {
+		sincos_octaves<6>(p, o, o + 6);
+	}
+}
+
+template <int ENC> __device__ __forceinline__ void encode_block(const float *rec, int q, float *o) {
+	if constexpr (ENC == NRC_ENCODING_UV_SINCOS16) {
+		float s[16], c[16];
+		sincos_octaves<16>(rec[q >> 1], s, c);
+#pragma unroll
+		for (int k = 0; k < 16; ++k) o[k] = (q & 1) ? c[k] : s[k];
+	} else {
+		float t[12];
+		if (q == 0) {
+			axis12<ENC>(rec[0], t);
+#pragma unroll
+			for (int k = 0; k < 12; ++k) o[k] = t[k];
+			axis12<ENC>(rec[1], t);
+#pragma unroll
+			for (int k = 0; k < 4; ++k) o[12 + k] = t[k];
+		} else if (q == 1) {
+			axis12<ENC>(rec[1], t);
+#pragma unroll
+			for (int k = 0; k < 8; ++k) o[k] = t[4 + k];
+			axis12<ENC>(rec[2], t);
+#pragma unroll
+			for (int k = 0; k < 8; ++k) o[8 + k] = t[k];
+		} else if (q == 2) {
+			axis12<ENC>(rec[2], t);
+#pragma unroll
+			for (int k = 0; k < 4; ++k) o[k] = t[8 + k];
+			oneblob4(rec[3], o + 4);
+			oneblob4(rec[4], o + 8);
+			oneblob4(rec[5], o + 12);
+		} else {
+			oneblob4(rec[6], o);
+			oneblob4(1.0f - expf(-rec[7]), o + 4);
+#pragma unroll
+			for (int k = 0; k < 6; ++k) o[8 + k] = rec[8 + k];
+			o[14] = 1.0f;
+			o[15] = 1.0f;
+		}
+	}
+}
+
+// All 64 features of one record.
+template <int ENC> __device__ __forceinline__ void encode_all(const float *rec, float *o) {
+	if constexpr (ENC == NRC_ENCODING_UV_SINCOS16) {
+		float s[16], c[16];
+#pragma unroll
+		for (int a = 0; a < 2; ++a) {
+			sincos_octaves<16>(rec[a], s, c);
+#pragma unroll
+			for (int k = 0; k < 16; ++k) {
+				o[32 * a + k] = s[k];
+				o[32 * a + 16 + k] = c[k];
+			}
+		}
+	} else {
+		axis12<ENC>(rec[0], o);
+		axis12<ENC>(rec[1], o + 12);
+		axis12<ENC>(rec[2], o + 24);
+		oneblob4(rec[3], o + 36);
+		oneblob4(rec[4], o + 40);
+		oneblob4(rec[5], o + 44);
+		oneblob4(rec[6], o + 48);
+		oneblob4(1.0f - expf(-rec[7]), o + 52);
+#pragma unroll
+		for (int k = 0; k < 6; ++k) o[56 + k] = rec[8 + k];
+		o[62] = 1.0f;
+		o[63] = 1.0f;
+	}
+}
+
+// NRCSphEncode, NRCRecord.glsl:47-49.
+__device__ __forceinline__ void sph_encode(float x, float y, float z, float *uv) {
+	constexpr float kPi = 3.14159265358979323846f;
+	uv[0] = (x == 0.0f && y == 0.0f) ? 0.5f : 0.5f + atan2f(y, x) / (2.0f * kPi);
+	uv[1] = acosf(fminf(fmaxf(z, -1.0f), 1.0f)) / kPi;
+}
+
+// ReLU backward with the reference's NaN-mask semantics (NN_nv.glsl:198-220,
+// 240-242): inactive unit or NaN result -> 0; +-Inf on an active unit passes.
+__device__ __forceinline__ float relu_bwd(float v, float act) { return (act > 0.0f && v == v) ? v : 0.0f; }
+
+// Per-sample loss gradient and value, NN_nv.glsl:162-196 (scale s; the
+// relative-L2 denominator is held constant).
+template <int LOSS>
+__device__ __forceinline__ float loss_grad(const float *o, const float *t, float s, float *g) {
+	float d = 1.0f;
+	if constexpr (LOSS == NRC_LOSS_RELATIVE_L2_LUMINANCE) {
+		float lum = 0.299f * fmaxf(o[0], 0.0f) + 0.587f * fmaxf(o[1], 0.0f) + 0.114f * fmaxf(o[2], 0.0f);
+		d = lum * lum + 0.01f;
+	}
+	float v = 0.0f;
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		float e = o[c] - t[c];
+		g[c] = 2.0f * s * e / d;
+		v += e * e / d;
+	}
+	return v;
+}
+
+} // namespace nrc

@@ -1,0 +1,75 @@
+// nrc_internal.hpp -- host-side launchers shared between the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nrc.h"
+#include "nrc_device.hpp"
+
+namespace nrc {
+
+// Samples per chunk of the plain weight-gradient reduction.
+constexpr uint32_t kPlainChunk = 512;
+// Samples per workgroup of the fused gradient kernel.
+constexpr uint32_t kFusedTrainSamples = 64;
+
+struct TrainArgs {
+	int hidden;
+	int encoding;
+	int loss;
+	const float *records; // [max_count][17]
+	const uint32_t *d_count;
+	uint32_t max_count;
+	uint32_t clamp; // train batch size
+	const float *w32;
+	const _Float16 *w16;
+	DeviceState *state;
+	float *partials; // [nparts][pstride]
+	uint32_t pstride;
+	// plain-path scratch, feature-major [(hidden+1)][64][ldn]
+	float *acts, *deltas, *sample_loss;
+	uint32_t ldn;
+};
+
+struct AdamArgs {
+	const float *partials;
+	uint32_t nparts, pstride;
+	const uint32_t *d_count;
+	uint32_t max_count, clamp;
+	DeviceState *state;
+	nrc_optimizer_entry *entries;
+	_Float16 *w16, *use16;
+	float *w32, *use32;
+	int write_use;
+	uint32_t weight_count;
+	AdamParams p;
+};
+
+// nrc_plain.hip
+hipError_t launch_plain_infer(int hidden, int encoding, const float *queries, const uint32_t *d_count,
+                              uint32_t max_count, const float *w32, float *rgb, hipStream_t stream);
+// Forward + backward per record, then the chunked dW reduction; returns the number of partials written.
+hipError_t launch_plain_gradient(const TrainArgs &a, uint32_t *nparts, hipStream_t stream);
+hipError_t launch_encode(int encoding, const float *queries, uint32_t n, float *features, hipStream_t stream);
+hipError_t launch_sph_encode(const float *dirs, uint32_t n, float *uv, hipStream_t stream);
+
+// nrc_optim.hip
+hipError_t launch_adam(const AdamArgs &a, hipStream_t stream);
+hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32_t pstride, uint32_t n, float *out,
+                                  hipStream_t stream);
+hipError_t launch_init_state(DeviceState *st, float loss_scale, hipStream_t stream);
+
+// nrc_fused.hip
+hipError_t launch_fused_infer(int hidden, int encoding, const float *queries, const uint32_t *d_count,
+                              uint32_t max_count, const _Float16 *w16, float *rgb, hipStream_t stream);
+hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *w16, float *out,
+                                    hipStream_t stream);
+hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t *nparts, hipStream_t stream);
+hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,
+                                     const _Float16 *w16, float *partials, uint32_t pstride, uint32_t *nparts,
+                                     hipStream_t stream);
+
+int device_cu_count();
+
+} // namespace nrc

@@ -1,0 +1,302 @@
+"""Python mirror of the NRC state interface over the C ABI in include/nrc.h.
+
+``NRCState`` follows the reference's ``VkNRCState`` (src/VkNRCState.hpp:17-90):
+construction builds and initialises the cache, ``reset_mlp_buffers`` is
+ResetMLPBuffers, ``set_use_ema_weights`` is SetUseEMAWeights, and the static
+sizes match GetWeightCount / GetTrainBatchCount / GetTrainBatchSize.  ``infer``
+replaces the NNDispatch<NNInference> pass and ``train`` the NNTrain pass group.
+
+Device buffers are anything with ``data_ptr()`` (torch tensors on the HIP device)
+or raw integer addresses.  The library is loaded from this directory; a missing
+library raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvknrc.so")
+
+WIDTH = 64
+OUTPUT_WIDTH = 3
+QUERY_FLOATS = 14
+TRAIN_RECORD_FLOATS = 17
+TRAIN_BATCH_COUNT = 4
+DEFAULT_TRAIN_BATCH_SIZE = 16384
+MAX_TRAIN_BATCH_SIZE = 65536
+
+PATH_PLAIN_F32, PATH_FUSED_F16 = 0, 1
+ENCODING_TRI12, ENCODING_SINCOS6, ENCODING_UV_SINCOS16 = 0, 1, 2
+LOSS_RELATIVE_L2_LUMINANCE, LOSS_L2 = 0, 1
+INIT_KAIMING_NORMAL, INIT_XAVIER_UNIFORM = 0, 1
+TRAIN_WRITE_USE_WEIGHTS = 1
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "HIP error",
+          5: "capacity exceeded"}
+
+
+class NRCError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class CacheDesc(C.Structure):
+    _fields_ = [("hidden_layers", C.c_uint32), ("encoding", C.c_int), ("loss", C.c_int), ("init", C.c_int),
+                ("seed", C.c_uint64), ("train_batch_size", C.c_uint32), ("learning_rate", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float), ("ema_alpha", C.c_float),
+                ("loss_scale", C.c_float), ("dynamic_loss_scale", C.c_int32),
+                ("loss_scale_growth_interval", C.c_uint32), ("use_ema_weights", C.c_int32)]
+
+
+class OptimizerState(C.Structure):
+    _fields_ = [("t", C.c_uint32), ("beta1_t", C.c_float), ("beta2_t", C.c_float), ("alpha_t", C.c_float),
+                ("alpha_t_1", C.c_float), ("loss_scale", C.c_float), ("clean_steps", C.c_uint32),
+                ("skipped_steps", C.c_uint32), ("last_loss", C.c_float), ("last_count", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libvknrc.so (raises when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run vknrc_amd.build.build() first")
+    L = C.CDLL(LIB_PATH)
+    P, U32, I32 = C.c_void_p, C.c_uint32, C.c_int32
+    L.nrc_status_string.restype = C.c_char_p
+    L.nrc_status_string.argtypes = [C.c_int]
+    L.nrc_last_error.restype = C.c_char_p
+    L.nrc_cache_desc_default.argtypes = [C.POINTER(CacheDesc)]
+    L.nrc_cache_desc_default.restype = None
+    L.nrc_weight_count.restype = U32
+    L.nrc_weight_count.argtypes = [U32]
+    L.nrc_init_weights.argtypes = [C.POINTER(CacheDesc), C.c_uint64, P]
+    L.nrc_cache_create.argtypes = [C.POINTER(CacheDesc), C.POINTER(P)]
+    L.nrc_cache_reset.argtypes = [P, C.c_uint64]
+    L.nrc_cache_destroy.argtypes = [P]
+    L.nrc_cache_destroy.restype = None
+    L.nrc_cache_get_desc.argtypes = [P, C.POINTER(CacheDesc)]
+    L.nrc_set_use_ema.argtypes = [P, I32]
+    L.nrc_infer.argtypes = [P, P, P, U32, P, C.c_int, P]
+    L.nrc_train.argtypes = [P, P, P, U32, C.c_int, U32, P]
+    L.nrc_frame.argtypes = [P, P, P, U32, P, P, P, U32, C.c_int, P]
+    L.nrc_get_state.argtypes = [P, P, C.POINTER(OptimizerState), P, P]
+    L.nrc_set_state.argtypes = [P, P, C.POINTER(OptimizerState)]
+    L.nrc_encode.argtypes = [C.c_int, P, U32, P, P]
+    L.nrc_sph_encode.argtypes = [P, U32, P, P]
+    L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
+    L.nrc_mlp_gradient_f16.argtypes = [U32, P, P, P, U32, P, P]
+    for f in ("nrc_init_weights", "nrc_cache_create", "nrc_cache_reset", "nrc_cache_get_desc", "nrc_set_use_ema",
+              "nrc_infer", "nrc_train", "nrc_frame", "nrc_get_state", "nrc_set_state", "nrc_encode",
+              "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16"):
+        getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise NRCError(status, load_library().nrc_last_error().decode())
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    return C.c_void_p(x.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except ImportError:
+            pass
+        return None
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+def weight_count(hidden_layers: int = 5) -> int:
+    return hidden_layers * WIDTH * WIDTH + OUTPUT_WIDTH * WIDTH
+
+
+@dataclass
+class NRCConfig:
+    """Cache description; defaults are the reference constants (Constant.glsl:5-13,
+    nrc_optimize.comp:10-11, VkNRCState.hpp:22-25)."""
+    hidden_layers: int = 5
+    encoding: int = ENCODING_TRI12
+    loss: int = LOSS_RELATIVE_L2_LUMINANCE
+    init: int = INIT_KAIMING_NORMAL
+    seed: int = 0
+    train_batch_size: int = DEFAULT_TRAIN_BATCH_SIZE
+    learning_rate: float = 0.002
+    beta1: float = 0.9
+    beta2: float = 0.999
+    epsilon: float = 1e-8
+    ema_alpha: float = 0.99
+    loss_scale: float = 1.0
+    dynamic_loss_scale: bool = False
+    loss_scale_growth_interval: int = 2000
+    use_ema_weights: bool = False
+
+    def to_c(self) -> CacheDesc:
+        d = CacheDesc()
+        for k, _ in CacheDesc._fields_:
+            setattr(d, k, int(getattr(self, k)) if isinstance(getattr(self, k), bool) else getattr(self, k))
+        return d
+
+
+def default_config() -> NRCConfig:
+    """The C library's defaults, read back through nrc_cache_desc_default."""
+    d = CacheDesc()
+    load_library().nrc_cache_desc_default(C.byref(d))
+    kw = {k: getattr(d, k) for k, _ in CacheDesc._fields_}
+    kw["dynamic_loss_scale"] = bool(kw["dynamic_loss_scale"])
+    kw["use_ema_weights"] = bool(kw["use_ema_weights"])
+    return NRCConfig(**kw)
+
+
+def init_weights(config: NRCConfig, seed: int | None = None) -> np.ndarray:
+    """Host-only: the FP32 initial weights of a cache built from ``config``."""
+    w = np.zeros(weight_count(config.hidden_layers), np.float32)
+    d = config.to_c()
+    _check(load_library().nrc_init_weights(C.byref(d), int(config.seed if seed is None else seed),
+                                           w.ctypes.data_as(C.c_void_p)))
+    return w
+
+
+class NRCState:
+    """One neural radiance cache: weights, use-weights and optimizer state on the device."""
+
+    TRAIN_BATCH_COUNT = TRAIN_BATCH_COUNT
+
+    def __init__(self, config: NRCConfig | None = None):
+        self.config = config or NRCConfig()
+        self._lib = load_library()
+        self._desc = self.config.to_c()
+        h = C.c_void_p()
+        _check(self._lib.nrc_cache_create(C.byref(self._desc), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.nrc_cache_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- VkNRCState interface
+    @staticmethod
+    def get_weight_count(hidden_layers: int = 5) -> int:
+        return weight_count(hidden_layers)
+
+    @staticmethod
+    def get_train_batch_count() -> int:
+        return TRAIN_BATCH_COUNT
+
+    def get_train_batch_size(self) -> int:
+        return self.config.train_batch_size
+
+    @property
+    def weight_count(self) -> int:
+        return weight_count(self.config.hidden_layers)
+
+    def reset_mlp_buffers(self, seed: int | None = None) -> None:
+        _check(self._lib.nrc_cache_reset(self._h, int(self.config.seed if seed is None else seed)))
+
+    def set_use_ema_weights(self, use_ema: bool) -> None:
+        self.config.use_ema_weights = bool(use_ema)
+        _check(self._lib.nrc_set_use_ema(self._h, int(bool(use_ema))))
+
+    def is_use_ema_weights(self) -> bool:
+        return self.config.use_ema_weights
+
+    # ---- NN passes
+    def infer(self, queries, out, count=None, max_count: int | None = None, path: int = PATH_FUSED_F16,
+              stream=None) -> None:
+        n = int(max_count if max_count is not None else out.shape[0])
+        _check(self._lib.nrc_infer(self._h, _ptr(queries), _ptr(count), n, _ptr(out), int(path), _stream(stream)))
+
+    def train(self, records, count=None, max_count: int | None = None, path: int = PATH_FUSED_F16,
+              write_use_weights: bool = True, stream=None) -> None:
+        n = int(max_count if max_count is not None else records.shape[0])
+        flags = TRAIN_WRITE_USE_WEIGHTS if write_use_weights else 0
+        _check(self._lib.nrc_train(self._h, _ptr(records), _ptr(count), n, int(path), flags, _stream(stream)))
+
+    def frame(self, queries, out, batches, query_count=None, batch_counts=None, max_queries=None,
+              max_batch_records=None, path: int = PATH_FUSED_F16, stream=None) -> None:
+        nq = int(max_queries if max_queries is not None else out.shape[0])
+        nb = int(max_batch_records if max_batch_records is not None else batches.shape[0] // TRAIN_BATCH_COUNT)
+        _check(self._lib.nrc_frame(self._h, _ptr(queries), _ptr(query_count), nq, _ptr(out), _ptr(batches),
+                                   _ptr(batch_counts), nb, int(path), _stream(stream)))
+
+    # ---- state export / import (synchronous)
+    def get_state(self) -> dict:
+        wc = self.weight_count
+        entries = np.zeros((wc, 4), np.float32)
+        w16 = np.zeros(wc, np.uint16)
+        u16 = np.zeros(wc, np.uint16)
+        st = OptimizerState()
+        _check(self._lib.nrc_get_state(self._h, entries.ctypes.data_as(C.c_void_p), C.byref(st),
+                                       w16.ctypes.data_as(C.c_void_p), u16.ctypes.data_as(C.c_void_p)))
+        return {"m": entries[:, 0].copy(), "v": entries[:, 1].copy(), "weight": entries[:, 2].copy(),
+                "ema_weight": entries[:, 3].copy(), "weights_f16": w16.view(np.float16).copy(),
+                "use_weights_f16": u16.view(np.float16).copy(), "state": st.as_dict()}
+
+    def set_state(self, weight=None, m=None, v=None, ema_weight=None, state: dict | None = None) -> None:
+        wc = self.weight_count
+        ent = None
+        if weight is not None:
+            ent = np.zeros((wc, 4), np.float32)
+            ent[:, 2] = weight
+            ent[:, 0] = 0 if m is None else m
+            ent[:, 1] = 0 if v is None else v
+            ent[:, 3] = weight if ema_weight is None else ema_weight
+        st = None
+        if state is not None:
+            st = OptimizerState()
+            for k, _ in OptimizerState._fields_:
+                if k in state:
+                    setattr(st, k, state[k])
+        _check(self._lib.nrc_set_state(self._h, None if ent is None else ent.ctypes.data_as(C.c_void_p),
+                                       None if st is None else C.byref(st)))
+
+
+# ---- stateless entry points
+def encode(queries, n: int, features, encoding: int = ENCODING_TRI12, stream=None) -> None:
+    _check(load_library().nrc_encode(int(encoding), _ptr(queries), int(n), _ptr(features), _stream(stream)))
+
+
+def sph_encode(dirs, n: int, uv, stream=None) -> None:
+    _check(load_library().nrc_sph_encode(_ptr(dirs), int(n), _ptr(uv), _stream(stream)))
+
+
+def mlp_forward_f16(weights_f16, inputs_f16, n: int, out, hidden_layers: int = 5, stream=None) -> None:
+    _check(load_library().nrc_mlp_forward_f16(int(hidden_layers), _ptr(weights_f16), _ptr(inputs_f16), int(n),
+                                              _ptr(out), _stream(stream)))
+
+
+def mlp_gradient_f16(weights_f16, inputs_f16, targets, n: int, dw, hidden_layers: int = 5, stream=None) -> None:
+    _check(load_library().nrc_mlp_gradient_f16(int(hidden_layers), _ptr(weights_f16), _ptr(inputs_f16),
+                                               _ptr(targets), int(n), _ptr(dw), _stream(stream)))

@@ -1,0 +1,127 @@
+"""Seeded synthetic workloads for the NRC hot path (queries, training records, targets).
+
+The reference's records come from its path tracer (path_tracer.comp, out of
+scope here); these generators produce records of the same layout
+(UnpackedNRCInput, NRCRecord.glsl:40-45 -- see include/nrc.h) with analytic
+targets, identical for the CPU oracle and the GPU paths.
+
+  queries          position U[-1,1]^3; scattered direction and normal uniform on
+                   the sphere, sphere-encoded to [0,1]^2 (NRCSphEncode); roughness
+                   U(0,1); diffuse and specular albedo U(0,1)^3
+  Gaussian lights  L(x) = sum_j I_j exp(-1/2 (x-mu_j)^T S_j^-1 (x-mu_j)),
+                   I_j ~ LogNormal(0, 1.5), anisotropic S_j = R diag(s^2) R^T;
+                   target = L(x) * (diffuse + specular) / 2
+  image            256x256 RGB: squash of 32 random 2-D sinusoids with
+                   frequencies U(1,32); records carry uv in position[0:2] and the
+                   bilinear (clamp-to-edge) sample as target
+"""
+from __future__ import annotations
+
+import numpy as np
+
+QUERY_FLOATS = 14
+TRAIN_FLOATS = 17
+MASTER_SEED = 0x4E52430000000000
+
+
+def config_seed(index: int, sub: int = 0) -> int:
+    return (MASTER_SEED + (index << 16) + sub) & 0xFFFFFFFFFFFFFFFF
+
+
+def _sphere(rng, n):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v
+
+
+def sph_encode(d: np.ndarray) -> np.ndarray:
+    """NRCSphEncode (NRCRecord.glsl:47-49) in numpy."""
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    u = np.where((x == 0) & (y == 0), 0.5, 0.5 + np.arctan2(y, x) / (2 * np.pi))
+    v = np.arccos(np.clip(z, -1, 1)) / np.pi
+    return np.stack([u, v], 1)
+
+
+def make_queries(n: int, rng: np.random.Generator) -> np.ndarray:
+    q = np.empty((n, QUERY_FLOATS), np.float32)
+    q[:, 0:3] = rng.uniform(-1.0, 1.0, (n, 3))
+    q[:, 3:5] = sph_encode(_sphere(rng, n))
+    q[:, 5:7] = sph_encode(_sphere(rng, n))
+    q[:, 7] = rng.uniform(0.0, 1.0, n)
+    q[:, 8:14] = rng.uniform(0.0, 1.0, (n, 6))
+    return q
+
+
+class GaussianLights:
+    def __init__(self, seed: int, count: int = 16):
+        rng = np.random.default_rng(seed)
+        self.mu = rng.uniform(-1.0, 1.0, (count, 3))
+        self.intensity = rng.lognormal(0.0, 1.5, count)
+        inv = []
+        for _ in range(count):
+            a = rng.normal(size=(3, 3))
+            qm, _ = np.linalg.qr(a)
+            s = rng.uniform(0.15, 0.6, 3)
+            inv.append(qm @ np.diag(1.0 / (s * s)) @ qm.T)
+        self.inv_cov = np.array(inv)
+
+    def radiance(self, pos: np.ndarray, chunk: int = 1 << 18) -> np.ndarray:
+        out = np.empty(pos.shape[0], np.float64)
+        for i in range(0, pos.shape[0], chunk):
+            p = pos[i:i + chunk].astype(np.float64)
+            d = p[:, None, :] - self.mu[None, :, :]
+            q = np.einsum("njk,jkl,njl->nj", d, self.inv_cov, d)
+            out[i:i + chunk] = (self.intensity[None, :] * np.exp(-0.5 * q)).sum(1)
+        return out
+
+    def targets(self, q: np.ndarray) -> np.ndarray:
+        L = self.radiance(q[:, 0:3])
+        return (L[:, None] * 0.5 * (q[:, 8:11] + q[:, 11:14])).astype(np.float32)
+
+
+def make_train_records(n: int, rng: np.random.Generator, lights: GaussianLights) -> np.ndarray:
+    q = make_queries(n, rng)
+    r = np.empty((n, TRAIN_FLOATS), np.float32)
+    r[:, :QUERY_FLOATS] = q
+    r[:, QUERY_FLOATS:] = lights.targets(q)
+    return r
+
+
+def make_image(seed: int, size: int = 256, waves: int = 32) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    fx = rng.uniform(1, 32, (3, waves))
+    fy = rng.uniform(1, 32, (3, waves))
+    amp = rng.uniform(0, 1, (3, waves))
+    ph = rng.uniform(0, 2 * np.pi, (3, waves))
+    t = (np.arange(size) + 0.5) / size
+    u, v = np.meshgrid(t, t, indexing="xy")
+    img = np.empty((size, size, 3), np.float64)
+    for c in range(3):
+        s = np.zeros_like(u)
+        for k in range(waves):
+            s += amp[c, k] * np.sin(2 * np.pi * (fx[c, k] * u + fy[c, k] * v) + ph[c, k])
+        img[:, :, c] = 0.5 + 0.5 * s / amp[c].sum()
+    return img.astype(np.float32)
+
+
+def sample_bilinear(img: np.ndarray, uv: np.ndarray) -> np.ndarray:
+    h, w, _ = img.shape
+    x = np.clip(uv[:, 0] * w - 0.5, 0, w - 1)
+    y = np.clip(uv[:, 1] * h - 0.5, 0, h - 1)
+    x0 = np.floor(x).astype(np.int64)
+    y0 = np.floor(y).astype(np.int64)
+    x1 = np.minimum(x0 + 1, w - 1)
+    y1 = np.minimum(y0 + 1, h - 1)
+    fx = (x - x0)[:, None]
+    fy = (y - y0)[:, None]
+    top = img[y0, x0] * (1 - fx) + img[y0, x1] * fx
+    bot = img[y1, x0] * (1 - fx) + img[y1, x1] * fx
+    return (top * (1 - fy) + bot * fy).astype(np.float32)
+
+
+def make_image_records(n: int, rng: np.random.Generator, img: np.ndarray) -> np.ndarray:
+    r = np.zeros((n, TRAIN_FLOATS), np.float32)
+    uv = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    r[:, 0:2] = uv
+    r[:, QUERY_FLOATS:] = sample_bilinear(img, uv.astype(np.float64))
+    return r

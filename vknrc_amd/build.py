@@ -1,5 +1,8 @@
 """Builds vknrc_amd/libvknrc.so (the C-ABI library of include/nrc.h) with hipcc for gfx950.
 
+Each source compiles to its own object in parallel, then hipcc links the shared
+library.
+
     python -m vknrc_amd.build            # or vknrc_amd.build.build()
 """
 from __future__ import annotations
@@ -7,6 +10,8 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -20,28 +25,40 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
 
+def _run(cmd):
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in l)
+    return res.returncode, out
+
+
 def build(verbose: bool = False) -> str:
     """Compile every HIP source for gfx950 into one shared library; returns its path."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    errs = "\n".join(l for l in (res.stdout + res.stderr).splitlines()
-                     if "packed-fp32-ops" not in l)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
-    if verbose and errs.strip():
-        print(errs, file=sys.stderr)
-    os.replace(tmp, LIB)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    with tempfile.TemporaryDirectory(prefix="vknrc_build_") as tmp:
+        jobs = []
+        for src in SOURCES:
+            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
+            jobs.append((obj, [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj]))
+        with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+            results = list(ex.map(lambda j: _run(j[1]), jobs))
+        for (obj, cmd), (rc, out) in zip(jobs, results):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            if rc != 0:
+                raise RuntimeError(f"hipcc failed on {cmd[-3]} ({rc}):\n{out}")
+            if verbose and out.strip():
+                print(out, file=sys.stderr)
+        tmp_lib = LIB + ".tmp"
+        rc, out = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for o, _ in jobs], "-o", tmp_lib])
+        if rc != 0:
+            raise RuntimeError(f"link failed ({rc}):\n{out}")
+        os.replace(tmp_lib, LIB)
     return LIB
 
 

@@ -18,7 +18,7 @@ struct nrc_cache {
 	int device = 0;
 	uint32_t wc = 0;
 	nrc_optimizer_entry *entries = nullptr;
-	_Float16 *w16 = nullptr, *use16 = nullptr;
+	_Float16 *w16 = nullptr, *w16t = nullptr, *use16 = nullptr;
 	float *w32 = nullptr, *use32 = nullptr;
 	DeviceState *state = nullptr;
 	float *partials = nullptr;
@@ -98,17 +98,21 @@ void host_init_weights(const nrc_cache_desc *d, uint64_t seed, float *w) {
 
 // Uploads entries + the derived fp16/fp32 weight copies (ResetMLPBuffers, VkNRCState.cpp:46-88).
 nrc_status upload_entries(nrc_cache *c, const nrc_optimizer_entry *entries) {
-	std::vector<_Float16> w16(c->wc), u16(c->wc);
+	std::vector<_Float16> w16(c->wc), w16t(c->wc), u16(c->wc);
 	std::vector<float> w32(c->wc), u32(c->wc);
+	const uint32_t hid = c->desc.hidden_layers * 4096u;
 	for (uint32_t i = 0; i < c->wc; ++i) {
 		w32[i] = entries[i].weight;
 		u32[i] = c->desc.use_ema_weights ? entries[i].ema_weight : entries[i].weight;
 		w16[i] = (_Float16)w32[i];
 		u16[i] = (_Float16)u32[i];
+		// transposed hidden layers [l][in][out] for the fused backward
+		w16t[i < hid ? ((i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)) : i] = w16[i];
 	}
 	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
 	NRC_HIP(hipMemcpy(c->entries, entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyHostToDevice), "upload entries");
 	NRC_HIP(hipMemcpy(c->w16, w16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->w16t, w16t.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->use16, u16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload use weights");
 	NRC_HIP(hipMemcpy(c->w32, w32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->use32, u32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload use weights");
@@ -131,7 +135,7 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 void free_cache(nrc_cache *c) {
 	if (!c) return;
 	(void)hipSetDevice(c->device);
-	void *ptrs[] = {c->entries, c->w16, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
+	void *ptrs[] = {c->entries, c->w16, c->w16t, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	delete c;
@@ -224,6 +228,7 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	hipError_t e = hipGetDevice(&c->device);
 	if (e == hipSuccess) e = hipMalloc(&c->entries, sizeof(nrc_optimizer_entry) * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w16, 2 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->w16t, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
@@ -297,12 +302,14 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	a.clamp = c->desc.train_batch_size;
 	a.w32 = c->w32;
 	a.w16 = c->w16;
+	a.w16t = c->w16t;
 	a.state = c->state;
 	a.partials = c->partials;
 	a.pstride = c->pstride;
 	// partial-gradient slots the launch will write, checked before anything runs
 	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
-	const uint32_t per = path == NRC_PATH_PLAIN_F32 ? nrc::kPlainChunk : nrc::kFusedTrainSamples;
+	const uint32_t chunks = path == NRC_PATH_PLAIN_F32 ? 1u : nrc::fused_train_chunks(n);
+	const uint32_t per = path == NRC_PATH_PLAIN_F32 ? nrc::kPlainChunk : nrc::kFusedTrainSamples * chunks;
 	const uint32_t need = (n + per - 1) / per;
 	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
 	uint32_t nparts = 0;
@@ -316,7 +323,7 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 		a.ldn = c->ldn;
 		e = nrc::launch_plain_gradient(a, &nparts, s);
 	} else {
-		e = nrc::launch_fused_gradient(a, &nparts, s);
+		e = nrc::launch_fused_gradient(a, chunks, &nparts, s);
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
 	nrc::AdamArgs o{};
@@ -329,11 +336,13 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	o.state = c->state;
 	o.entries = c->entries;
 	o.w16 = c->w16;
+	o.w16t = c->w16t;
 	o.use16 = c->use16;
 	o.w32 = c->w32;
 	o.use32 = c->use32;
 	o.write_use = (flags & NRC_TRAIN_WRITE_USE_WEIGHTS) ? 1 : 0;
 	o.weight_count = c->wc;
+	o.hidden = (int)c->desc.hidden_layers;
 	o.p = adam_params(c);
 	e = nrc::launch_adam(o, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
@@ -424,12 +433,18 @@ nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint
 	const uint32_t pstride = ((wc + 1) + 63u) & ~63u;
 	const uint32_t nb = n == 0 ? 1 : (n + nrc::kFusedTrainSamples - 1) / nrc::kFusedTrainSamples;
 	float *partials = nullptr;
+	_Float16 *wt = nullptr;
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s), "gradient scratch");
+	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wt), 2 * (size_t)wc, s), "gradient scratch");
 	uint32_t nparts = 0;
-	hipError_t e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n,
-	                                              reinterpret_cast<const _Float16 *>(d_w), partials, pstride, &nparts, s);
+	const _Float16 *w = reinterpret_cast<const _Float16 *>(d_w);
+	hipError_t e = nrc::launch_transpose_hidden((int)hidden, w, wt, s);
+	if (e == hipSuccess)
+		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, w, wt,
+		                                   partials, pstride, &nparts, s);
 	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
 	hipError_t e2 = hipFreeAsync(partials, s);
+	if (e2 == hipSuccess) e2 = hipFreeAsync(wt, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16");
 	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_gradient_f16 free");
 	return NRC_OK;

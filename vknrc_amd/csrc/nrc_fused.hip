@@ -16,6 +16,8 @@
 // Weight layout in HBM (unchanged from the reference, NN_nv.glsl:55-57): the
 // fp16 weights of layer l, row-major [out][in], at l*4096; the 3x64 output
 // layer at H*4096 (padded to 16 rows with zeros inside the MFMA, NN_nv.glsl:77-82).
+#include <cstdlib>
+
 #include "nrc_internal.hpp"
 
 namespace nrc {
@@ -33,22 +35,33 @@ __device__ __forceinline__ float4_t mfma(half8_t a, half8_t b, float4_t c) {
 	return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
+
+// Two FP32 values -> packed FP16 (round to nearest even), then ReLU on the pair.
+// max(cvt(x), 0) equals cvt(max(x, 0)) up to the sign of a zero.
+__device__ __forceinline__ half2_t cvt_relu2(float a, float b) {
+	const float2_t f = {a, b};
+	return __builtin_elementwise_max(__builtin_convertvector(f, half2_t), (half2_t)0);
+}
+
 __device__ __forceinline__ half8_t pack_relu(float4_t lo, float4_t hi) {
+	const half2_t a = cvt_relu2(lo[0], lo[1]), b = cvt_relu2(lo[2], lo[3]);
+	const half2_t c = cvt_relu2(hi[0], hi[1]), d = cvt_relu2(hi[2], hi[3]);
 	half8_t h;
-#pragma unroll
-	for (int i = 0; i < 4; ++i) {
-		h[i] = (_Float16)fmaxf(lo[i], 0.0f);
-		h[4 + i] = (_Float16)fmaxf(hi[i], 0.0f);
-	}
+	h[0] = a[0]; h[1] = a[1]; h[2] = b[0]; h[3] = b[1];
+	h[4] = c[0]; h[5] = c[1]; h[6] = d[0]; h[7] = d[1];
 	return h;
 }
 
 // Orders LDS traffic between lanes of one wavefront (LDS operations of a wave
-// execute in order; this keeps the compiler from moving them across).
+// execute in order; this keeps the compiler from moving them across).  The
+// fences are restricted to LDS so they wait on LDS counters only and leave the
+// prefetched global loads in flight.
 __device__ __forceinline__ void wave_lds_sync() {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
 	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Row-major [rows][64] fp16 weights -> padded LDS rows.
@@ -80,7 +93,7 @@ template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(c
 			for (int k0 = 0; k0 < 16; k0 += 4) {
 				float s[4], c[4];
 				float sv, cv;
-				sincospif(rec[a] * (float)(1 << k0), &sv, &cv);
+				sincospi_poly(rec[a] * (float)(1 << k0), &sv, &cv);
 				s[0] = sv;
 				c[0] = cv;
 #pragma unroll
@@ -161,41 +174,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 	half4_t *sx4 = reinterpret_cast<half4_t *>(sx);
 	const int et = lane >> 4, es = lane & 15; // this lane's record: tile et, column es
 
-	for (uint32_t tile = blockIdx.x * 4 + wave; tile < ntiles; tile += nwaves) {
-		const uint32_t base = tile * 64;
+	// the record of this lane in tile t.  The index is clamped into [0, count)
+	// instead of branching, so the load is unconditional (lanes past the count
+	// encode zeros regardless); a branch here makes the wait-count pass drain
+	// every outstanding load at the join.  count == 0 reads record 0, which
+	// exists because max_count > 0.
+	const uint32_t last = count ? count - 1 : 0;
+	auto load_query = [&](uint32_t t, float *dst) __attribute__((always_inline)) {
+		uint32_t m = t * 64 + lane;
+		m = m < last ? m : last;
+		const float2 *qp = reinterpret_cast<const float2 *>(queries + (size_t)m * kQueryFloats);
+#pragma unroll
+		for (int i = 0; i < 7; ++i) {
+			const float2 v = qp[i];
+			dst[2 * i] = v.x;
+			dst[2 * i + 1] = v.y;
+		}
+	};
+	// encode one record per lane straight into the B-fragment layout of this wave's slot
+	auto encode_tile = [&](uint32_t base, const float *rec) __attribute__((always_inline)) {
 		const uint32_t n = base + lane;
-		auto emit = [&](int p, float f0, float f1, float f2, float f3) {
+		auto emit = [&](int p, float f0, float f1, float f2, float f3) __attribute__((always_inline)) {
 			const int c = p >> 3, gg = p & 3, hi = (p >> 2) & 1;
 			half4_t h;
 			h[0] = (_Float16)f0; h[1] = (_Float16)f1; h[2] = (_Float16)f2; h[3] = (_Float16)f3;
 			sx4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = h;
 		};
-		if (n < count) {
-			if constexpr (F16IN) {
+		if constexpr (F16IN) {
+			if (n < count) {
 				const half4_t *src = reinterpret_cast<const half4_t *>(fin + (size_t)n * kWidth);
 #pragma unroll
 				for (int p = 0; p < 16; ++p) {
-					const half4_t v = src[p];
 					const int c = p >> 3, gg = p & 3, hi = (p >> 2) & 1;
-					sx4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = v;
+					sx4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = src[p];
 				}
 			} else {
-				float rec[kQueryFloats];
-				const float2 *qp = reinterpret_cast<const float2 *>(queries + (size_t)n * kQueryFloats);
 #pragma unroll
-				for (int i = 0; i < 7; ++i) {
-					const float2 v = qp[i];
-					rec[2 * i] = v.x;
-					rec[2 * i + 1] = v.y;
-				}
-				encode_quads<ENC>(rec, emit);
+				for (int p = 0; p < 16; ++p) emit(p, 0.0f, 0.0f, 0.0f, 0.0f);
 			}
 		} else {
+			if (n < count) {
+				encode_quads<ENC>(rec, emit);
+			} else {
 #pragma unroll
-			for (int p = 0; p < 16; ++p) emit(p, 0.0f, 0.0f, 0.0f, 0.0f);
+				for (int p = 0; p < 16; ++p) emit(p, 0.0f, 0.0f, 0.0f, 0.0f);
+			}
 		}
 		wave_lds_sync();
+	};
 
+	// the 6 layers over the 4 N-tiles of the staged tile, TILES at a time
+	auto mlp_tile = [&](uint32_t base) __attribute__((always_inline)) {
 #pragma unroll
 		for (int tg = 0; tg < 4; tg += TILES) {
 			half8_t B[TILES][2];
@@ -241,21 +270,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 			}
 		}
 		wave_lds_sync();
+	};
+
+	uint32_t tile = blockIdx.x * 4 + wave;
+	if constexpr (F16IN) {
+		for (; tile < ntiles; tile += nwaves) {
+			encode_tile(tile * 64, nullptr);
+			mlp_tile(tile * 64);
+		}
+	} else {
+		// two record buffers in ping-pong: the next tile's records are in flight
+		// while this tile is encoded and run, and no register copy waits on them
+		float ra[kQueryFloats], rb[kQueryFloats];
+		load_query(tile, ra);
+		for (; tile < ntiles; tile += 2 * nwaves) {
+			load_query(tile + nwaves, rb);
+			encode_tile(tile * 64, ra);
+			mlp_tile(tile * 64);
+			if (tile + nwaves >= ntiles) break;
+			load_query(tile + 2 * nwaves, ra);
+			encode_tile((tile + nwaves) * 64, rb);
+			mlp_tile((tile + nwaves) * 64);
+		}
 	}
 }
 
 // ================================================================ gradient
 //
-// One workgroup = 4 waves = 64 records (wave w owns N-tile w).  LDS holds the
-// fp16 weights row-major and transposed, every layer's activations
-// feature-major [feature][record] (the K = records operand layout of the dW
-// MFMAs), and the current layer's delta.
+// One workgroup = 4 waves; it walks `chunks` consecutive 64-record chunks (wave
+// w owns N-tile w of each chunk) and accumulates its part of dW in VGPRs.  LDS
+// holds the fp16 weights row-major and transposed (the transposed copy is
+// written by the optimizer, so staging is a straight copy), every layer's
+// activations feature-major [feature][record] -- the K = records operand
+// layout of the dW MFMAs -- and the current layer's delta.
 //   forward   : per wave, B operands in VGPRs, A operands from LDS
-//   dW_l      : wave w computes rows 16w..16w+15 of dW_l, K = the 64 records
+//   dW_l^T    : wave w owns output rows 16w..16w+15 of every layer; computing
+//               the transposed tile (A = a_l, B = delta_l) leaves 4 consecutive
+//               input columns per lane, so partials leave as float4 stores
 //   delta_l-1 : per wave, (W_l^T delta_l) . 1[a_l > 0] with A from the
 //               transposed copy and the ReLU mask from the saved B operands
-// The workgroup writes its partial dW (FP32) and loss sum; the optimizer kernel
-// reduces the partials in a fixed order.
+// The workgroup writes its partial dW (FP32) and loss sum once; the optimizer
+// kernel reduces the partials in a fixed order.
 template <int H> struct GradSmem {
 	_Float16 w[(H * kWidth + kOut) * kLdw];
 	_Float16 wt[H * kWidth * kLdw];
@@ -265,12 +320,17 @@ template <int H> struct GradSmem {
 	float loss[64];
 };
 
+// feature index of element j of a B operand held by lane group g for K-chunk c
+__device__ __forceinline__ int bfeat(int c, int g, int j) { return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4); }
+
 template <int H, int ENC, int LOSS, bool F16IN>
 __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict__ records,
                                                          const _Float16 *__restrict__ fin,
                                                          const float *__restrict__ ftgt,
                                                          const uint32_t *__restrict__ d_count, uint32_t max_count,
-                                                         uint32_t clamp, const _Float16 *__restrict__ w16,
+                                                         uint32_t clamp, uint32_t chunks,
+                                                         const _Float16 *__restrict__ w16,
+                                                         const _Float16 *__restrict__ w16t,
                                                          const DeviceState *__restrict__ st,
                                                          float *__restrict__ partials, uint32_t pstride,
                                                          uint32_t *__restrict__ overflow) {
@@ -282,183 +342,60 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 	const int g = lane >> 4, r = lane & 15;
 	const uint32_t count = F16IN ? max_count : effective_count(d_count, max_count, clamp);
 	const float scale = (F16IN || st == nullptr) ? 1.0f : st->pub.loss_scale;
-	const uint32_t base = blockIdx.x * 64u;
-	float *part = partials + (size_t)blockIdx.x * pstride;
+	const int col = 16 * wave + r; // this lane's record column inside a chunk
 
-	// ---- stage weights (row-major and transposed)
 	stage_rows(S.w, w16, H * kWidth + kOut);
-	for (int e = tid; e < H * kWidth * kWidth; e += 256) {
-		const int l = e >> 12, m = (e >> 6) & 63, k = e & 63;
-		S.wt[(l * kWidth + k) * kLdw + m] = w16[e];
-	}
-	// ---- encode: wave w computes feature quads 4w..4w+3 of all 64 records
-	{
-		const uint32_t n = base + lane;
-		const bool valid = n < count;
-		_Float16 *a0 = S.act;
-		if constexpr (F16IN) {
-#pragma unroll
-			for (int j = 0; j < 16; ++j)
-				a0[(16 * wave + j) * kLdn + lane] = valid ? fin[(size_t)(valid ? n : 0u) * kWidth + 16 * wave + j] : (_Float16)0.0f;
-			if (wave == 0)
-#pragma unroll
-				for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? ftgt[(size_t)(valid ? n : 0u) * 3 + c] : 0.0f;
-		} else {
-			float rec[kQueryFloats];
-			const float *rp = records + (size_t)(valid ? n : 0u) * kTrainFloats;
-#pragma unroll
-			for (int i = 0; i < kQueryFloats; ++i) rec[i] = valid ? rp[i] : 0.0f;
-			float f[16];
-			encode_block<ENC>(rec, wave, f);
-#pragma unroll
-			for (int j = 0; j < 16; ++j) a0[(16 * wave + j) * kLdn + lane] = valid ? (_Float16)f[j] : (_Float16)0.0f;
-			if (wave == 0)
-#pragma unroll
-				for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? rp[14 + c] : 0.0f;
-		}
-	}
-	__syncthreads();
+	stage_rows(S.wt, w16t, H * kWidth);
 
-	// ---- forward for this wave's 16 records
-	const int col = 16 * wave + r; // this lane's record column inside the workgroup
-	half8_t Bs[H + 1][2];           // fp16 activations a_0..a_H as B operands
+	float4_t accW[H][4]; // dW_l^T tiles: rows = input columns 16nt + 4g + i, column = output row 16w + r
+	float4_t accO = (float4_t)0.0f;
 #pragma unroll
-	for (int c = 0; c < 2; ++c)
+	for (int l = 0; l < H; ++l)
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const int k = 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-			Bs[0][c][j] = S.act[k * kLdn + col];
-		}
-#pragma unroll
-	for (int l = 0; l < H; ++l) {
-		float4_t acc[4];
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
-#pragma unroll
-		for (int c = 0; c < 2; ++c)
-#pragma unroll
-			for (int mt = 0; mt < 4; ++mt)
-				acc[mt] = mfma(load_afrag(S.w, l * kWidth + mt * 16 + r, c, g), Bs[l][c], acc[mt]);
-#pragma unroll
-		for (int c = 0; c < 2; ++c) Bs[l + 1][c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
-		_Float16 *al = S.act + (l + 1) * kWidth * kLdn;
-#pragma unroll
-		for (int c = 0; c < 2; ++c)
-#pragma unroll
-			for (int j = 0; j < 8; ++j) {
-				const int k = 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-				al[k * kLdn + col] = Bs[l + 1][c][j];
-			}
-	}
-	float4_t o = (float4_t)0.0f;
-#pragma unroll
-	for (int c = 0; c < 2; ++c) {
-		half8_t a = load_afrag(S.w, H * kWidth + (r < kOut ? r : 0), c, g);
-		if (r >= kOut) a = (half8_t)0;
-		o = mfma(a, Bs[H][c], o);
-	}
+		for (int nt = 0; nt < 4; ++nt) accW[l][nt] = (float4_t)0.0f;
+	float loss_sum = 0.0f;
 
-	// ---- loss gradient (lanes g == 0 hold the 3 outputs of record col)
-	half8_t Bg = (half8_t)0; // g_o as the B operand of the output-layer backward (K rows 0..2)
-	{
-		float go[3] = {0.0f, 0.0f, 0.0f};
-		if (g == 0) {
-			const bool valid = base + col < count;
-			const float oo[3] = {o[0], o[1], o[2]};
-			const float tt[3] = {S.tgt[col * 3], S.tgt[col * 3 + 1], S.tgt[col * 3 + 2]};
-			const float v = loss_grad<LOSS>(oo, tt, scale, go);
-			if (!valid) go[0] = go[1] = go[2] = 0.0f;
-			S.loss[col] = valid ? v : 0.0f;
-			Bg[0] = (_Float16)go[0];
-			Bg[1] = (_Float16)go[1];
-			Bg[2] = (_Float16)go[2];
-		}
-		// delta rows 0..15 for record col: rows 0..2 = g_o, the rest 0 (padded output rows)
-		_Float16 *d = S.delta;
-#pragma unroll
-		for (int i = 0; i < 4; ++i) d[(4 * g + i) * kLdn + col] = (g == 0 && i < 3) ? Bg[i] : (_Float16)0.0f;
-	}
-	__syncthreads();
-
-	bool bad = false;
-	// ---- dW_H (3 x 64): wave w computes input columns 16w..16w+15
-	{
-		float4_t acc = (float4_t)0.0f;
-#pragma unroll
-		for (int c = 0; c < 2; ++c) {
-			const half8_t a = *reinterpret_cast<const half8_t *>(S.delta + r * kLdn + 32 * c + 8 * g);
-			const half8_t b = *reinterpret_cast<const half8_t *>(S.act + (H * kWidth + 16 * wave + r) * kLdn + 32 * c + 8 * g);
-			acc = mfma(a, b, acc);
-		}
-		if (g == 0)
-#pragma unroll
-			for (int i = 0; i < kOut; ++i) {
-				part[H * kWidth * kWidth + i * kWidth + 16 * wave + r] = acc[i];
-				bad |= !isfinite(acc[i]);
-			}
-	}
-	// ---- delta_{H-1} = (W_H^T g_o) . 1[a_H > 0] for this wave's records
-	half8_t Bd[2];
-	{
-		float4_t acc[4];
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) {
-			half8_t a = (half8_t)0;
-			if (g == 0) {
-				const int m = 16 * mt + r;
-				a[0] = S.w[(H * kWidth + 0) * kLdw + m];
-				a[1] = S.w[(H * kWidth + 1) * kLdw + m];
-				a[2] = S.w[(H * kWidth + 2) * kLdw + m];
-			}
-			acc[mt] = mfma(a, Bg, (float4_t)0.0f);
-		}
-#pragma unroll
-		for (int c = 0; c < 2; ++c)
-#pragma unroll
-			for (int i = 0; i < 4; ++i) {
-				Bd[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)Bs[H][c][i]);
-				Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[H][c][4 + i]);
-			}
-	}
-	__syncthreads(); // every wave has finished reading delta rows of g_o
-#pragma unroll
-	for (int c = 0; c < 2; ++c)
-#pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const int k = 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-			S.delta[k * kLdn + col] = Bd[c][j];
-		}
-	__syncthreads();
-
-	// ---- layers H-1 .. 0
-#pragma unroll
-	for (int l = H - 1; l >= 0; --l) {
-		// dW_l rows 16*wave .. 16*wave+15, all 64 input columns
+	for (uint32_t ch = 0; ch < chunks; ++ch) {
+		const uint32_t base = (blockIdx.x * chunks + ch) * 64u;
+		if (base >= count) break; // uniform over the workgroup
+		__syncthreads();          // previous chunk's LDS readers are done
+		// ---- encode: wave w computes features 16w..16w+15 of all 64 records
 		{
-			float4_t acc[4];
+			const uint32_t n = base + lane;
+			const bool valid = n < count;
+			const uint32_t ns = valid ? n : 0u;
+			_Float16 *a0 = S.act;
+			if constexpr (F16IN) {
 #pragma unroll
-			for (int nt = 0; nt < 4; ++nt) acc[nt] = (float4_t)0.0f;
+				for (int j = 0; j < 16; ++j)
+					a0[(16 * wave + j) * kLdn + lane] = valid ? fin[(size_t)ns * kWidth + 16 * wave + j] : (_Float16)0.0f;
+				if (wave == 0)
 #pragma unroll
-			for (int c = 0; c < 2; ++c) {
-				const half8_t a = *reinterpret_cast<const half8_t *>(S.delta + (16 * wave + r) * kLdn + 32 * c + 8 * g);
+					for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? ftgt[(size_t)ns * 3 + c] : 0.0f;
+			} else {
+				float rec[kQueryFloats];
+				const float *rp = records + (size_t)ns * kTrainFloats;
 #pragma unroll
-				for (int nt = 0; nt < 4; ++nt) {
-					const half8_t b = *reinterpret_cast<const half8_t *>(S.act + (l * kWidth + 16 * nt + r) * kLdn + 32 * c + 8 * g);
-					acc[nt] = mfma(a, b, acc[nt]);
-				}
+				for (int i = 0; i < kQueryFloats; ++i) rec[i] = valid ? rp[i] : 0.0f;
+				float f[16];
+				encode_block<ENC>(rec, wave, f);
+#pragma unroll
+				for (int j = 0; j < 16; ++j) a0[(16 * wave + j) * kLdn + lane] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+				if (wave == 0)
+#pragma unroll
+					for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? rp[14 + c] : 0.0f;
 			}
-			float *pl = part + l * kWidth * kWidth;
-#pragma unroll
-			for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-				for (int i = 0; i < 4; ++i) {
-					pl[(16 * wave + 4 * g + i) * kWidth + 16 * nt + r] = acc[nt][i];
-					bad |= !isfinite(acc[nt][i]);
-				}
 		}
-		if (l == 0) break;
-		// delta_{l-1} = (W_l^T delta_l) . 1[a_l > 0]
-		{
+		__syncthreads();
+
+		// ---- forward for this wave's 16 records
+		half8_t Bs[H + 1][2]; // fp16 activations a_0..a_H as B operands
+#pragma unroll
+		for (int c = 0; c < 2; ++c)
+#pragma unroll
+			for (int j = 0; j < 8; ++j) Bs[0][c][j] = S.act[bfeat(c, g, j) * kLdn + col];
+#pragma unroll
+		for (int l = 0; l < H; ++l) {
 			float4_t acc[4];
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
@@ -466,32 +403,154 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt)
-					acc[mt] = mfma(load_afrag(S.wt, l * kWidth + mt * 16 + r, c, g), Bd[c], acc[mt]);
+					acc[mt] = mfma(load_afrag(S.w, l * kWidth + mt * 16 + r, c, g), Bs[l][c], acc[mt]);
+#pragma unroll
+			for (int c = 0; c < 2; ++c) Bs[l + 1][c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
+			_Float16 *al = S.act + (l + 1) * kWidth * kLdn;
+#pragma unroll
+			for (int c = 0; c < 2; ++c)
+#pragma unroll
+				for (int j = 0; j < 8; ++j) al[bfeat(c, g, j) * kLdn + col] = Bs[l + 1][c][j];
+		}
+		float4_t o = (float4_t)0.0f;
+#pragma unroll
+		for (int c = 0; c < 2; ++c) {
+			half8_t a = load_afrag(S.w, H * kWidth + (r < kOut ? r : 0), c, g);
+			if (r >= kOut) a = (half8_t)0;
+			o = mfma(a, Bs[H][c], o);
+		}
+
+		// ---- loss gradient (lanes g == 0 hold the 3 outputs of record col)
+		half8_t Bg = (half8_t)0; // g_o as the B operand of the output-layer backward (K rows 0..2)
+		{
+			float go[3] = {0.0f, 0.0f, 0.0f};
+			if (g == 0) {
+				const bool valid = base + col < count;
+				const float oo[3] = {o[0], o[1], o[2]};
+				const float tt[3] = {S.tgt[col * 3], S.tgt[col * 3 + 1], S.tgt[col * 3 + 2]};
+				const float v = loss_grad<LOSS>(oo, tt, scale, go);
+				if (!valid) go[0] = go[1] = go[2] = 0.0f;
+				S.loss[col] = valid ? v : 0.0f;
+				Bg[0] = (_Float16)go[0];
+				Bg[1] = (_Float16)go[1];
+				Bg[2] = (_Float16)go[2];
+			}
+			// delta rows 0..15 for record col: rows 0..2 = g_o, the rest 0 (padded output rows)
+#pragma unroll
+			for (int i = 0; i < 4; ++i) S.delta[(4 * g + i) * kLdn + col] = (g == 0 && i < 3) ? Bg[i] : (_Float16)0.0f;
+		}
+		__syncthreads();
+		if (tid == 0)
+			for (int i = 0; i < 64; ++i) loss_sum += S.loss[i];
+
+		// ---- dW_H^T: rows = input columns 16w + 4g + i, column = output row r (< 3 valid)
+#pragma unroll
+		for (int c = 0; c < 2; ++c) {
+			const half8_t a = *reinterpret_cast<const half8_t *>(S.act + (H * kWidth + 16 * wave + r) * kLdn + 32 * c + 8 * g);
+			const half8_t b = *reinterpret_cast<const half8_t *>(S.delta + r * kLdn + 32 * c + 8 * g);
+			accO = mfma(a, b, accO);
+		}
+		// ---- delta_{H-1} = (W_H^T g_o) . 1[a_H > 0] for this wave's records
+		half8_t Bd[2];
+		{
+			float4_t acc[4];
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) {
+				half8_t a = (half8_t)0;
+				if (g == 0) {
+					const int m = 16 * mt + r;
+					a[0] = S.w[(H * kWidth + 0) * kLdw + m];
+					a[1] = S.w[(H * kWidth + 1) * kLdw + m];
+					a[2] = S.w[(H * kWidth + 2) * kLdw + m];
+				}
+				acc[mt] = mfma(a, Bg, (float4_t)0.0f);
+			}
 #pragma unroll
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
 				for (int i = 0; i < 4; ++i) {
-					Bd[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)Bs[l][c][i]);
-					Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[l][c][4 + i]);
+					Bd[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)Bs[H][c][i]);
+					Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[H][c][4 + i]);
 				}
 		}
-		__syncthreads(); // dW_l reads of delta are done
+		__syncthreads(); // every wave has finished reading the g_o rows of delta
 #pragma unroll
 		for (int c = 0; c < 2; ++c)
 #pragma unroll
-			for (int j = 0; j < 8; ++j) {
-				const int k = 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-				S.delta[k * kLdn + col] = Bd[c][j];
-			}
+			for (int j = 0; j < 8; ++j) S.delta[bfeat(c, g, j) * kLdn + col] = Bd[c][j];
 		__syncthreads();
+
+		// ---- layers H-1 .. 0
+#pragma unroll
+		for (int l = H - 1; l >= 0; --l) {
+			// dW_l^T tiles for output rows 16w .. 16w+15, all 64 input columns
+#pragma unroll
+			for (int c = 0; c < 2; ++c) {
+				const half8_t b = *reinterpret_cast<const half8_t *>(S.delta + (16 * wave + r) * kLdn + 32 * c + 8 * g);
+#pragma unroll
+				for (int nt = 0; nt < 4; ++nt) {
+					const half8_t a =
+					    *reinterpret_cast<const half8_t *>(S.act + (l * kWidth + 16 * nt + r) * kLdn + 32 * c + 8 * g);
+					accW[l][nt] = mfma(a, b, accW[l][nt]);
+				}
+			}
+			if (l == 0) break;
+			// delta_{l-1} = (W_l^T delta_l) . 1[a_l > 0]
+			{
+				float4_t acc[4];
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
+#pragma unroll
+				for (int c = 0; c < 2; ++c)
+#pragma unroll
+					for (int mt = 0; mt < 4; ++mt)
+						acc[mt] = mfma(load_afrag(S.wt, l * kWidth + mt * 16 + r, c, g), Bd[c], acc[mt]);
+#pragma unroll
+				for (int c = 0; c < 2; ++c)
+#pragma unroll
+					for (int i = 0; i < 4; ++i) {
+						Bd[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)Bs[l][c][i]);
+						Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[l][c][4 + i]);
+					}
+			}
+			__syncthreads(); // dW_l reads of delta are done
+#pragma unroll
+			for (int c = 0; c < 2; ++c)
+#pragma unroll
+				for (int j = 0; j < 8; ++j) S.delta[bfeat(c, g, j) * kLdn + col] = Bd[c][j];
+			__syncthreads();
+		}
 	}
 
-	if (tid == 0) {
-		float s = 0.0f;
-		for (int i = 0; i < 64; ++i) s += S.loss[i];
-		part[WC] = s;
+	// ---- partial dW of this workgroup (float4 per lane and tile) and its loss sum
+	float *part = partials + (size_t)blockIdx.x * pstride;
+	bool bad = false;
+#pragma unroll
+	for (int l = 0; l < H; ++l)
+#pragma unroll
+		for (int nt = 0; nt < 4; ++nt) {
+			const float4_t v4 = accW[l][nt];
+			*reinterpret_cast<float4_t *>(part + l * kWidth * kWidth + (16 * wave + r) * kWidth + 16 * nt + 4 * g) = v4;
+#pragma unroll
+			for (int i = 0; i < 4; ++i) bad |= !isfinite(v4[i]);
+		}
+	if (r < kOut) {
+		*reinterpret_cast<float4_t *>(part + H * kWidth * kWidth + r * kWidth + 16 * wave + 4 * g) = accO;
+#pragma unroll
+		for (int i = 0; i < 4; ++i) bad |= !isfinite(accO[i]);
 	}
+	if (tid == 0) part[WC] = loss_sum;
 	if (bad && overflow) atomicOr(overflow, 1u);
+}
+
+// N-tiles each wave runs through the layers together (1 or 2); VKNRC_INFER_TILES overrides.
+int infer_tiles() {
+	static int t = 0;
+	if (t == 0) {
+		const char *e = getenv("VKNRC_INFER_TILES");
+		t = (e && atoi(e) == 1) ? 1 : 2;
+	}
+	return t;
 }
 
 template <int H, int ENC, bool F16IN>
@@ -502,14 +561,18 @@ hipError_t infer_launch(const float *q, const _Float16 *fin, const uint32_t *dc,
 	const uint32_t cap = (uint32_t)device_cu_count() * 2;
 	if (blocks > cap) blocks = cap;
 	if (blocks == 0) blocks = 1;
-	fused_infer_kernel<H, ENC, F16IN, 2><<<dim3(blocks), dim3(256), 0, s>>>(q, fin, dc, mc, w16, out, clamp0);
+	if (infer_tiles() == 1)
+		fused_infer_kernel<H, ENC, F16IN, 1><<<dim3(blocks), dim3(256), 0, s>>>(q, fin, dc, mc, w16, out, clamp0);
+	else
+		fused_infer_kernel<H, ENC, F16IN, 2><<<dim3(blocks), dim3(256), 0, s>>>(q, fin, dc, mc, w16, out, clamp0);
 	return hipGetLastError();
 }
 
 template <int H, int ENC, int LOSS, bool F16IN>
 hipError_t grad_launch(const float *rec, const _Float16 *fin, const float *ftgt, const uint32_t *dc, uint32_t mc,
-                       uint32_t clamp, const _Float16 *w16, const DeviceState *st, float *partials, uint32_t pstride,
-                       uint32_t *overflow, uint32_t nblocks, hipStream_t s) {
+                       uint32_t clamp, uint32_t chunks, const _Float16 *w16, const _Float16 *w16t,
+                       const DeviceState *st, float *partials, uint32_t pstride, uint32_t *overflow, uint32_t nblocks,
+                       hipStream_t s) {
 	const size_t smem = sizeof(GradSmem<H>);
 	auto kern = fused_grad_kernel<H, ENC, LOSS, F16IN>;
 	static bool attr_set = false;
@@ -519,25 +582,28 @@ hipError_t grad_launch(const float *rec, const _Float16 *fin, const float *ftgt,
 		if (e != hipSuccess) return e;
 		attr_set = true;
 	}
-	kern<<<dim3(nblocks), dim3(256), smem, s>>>(rec, fin, ftgt, dc, mc, clamp, w16, st, partials, pstride, overflow);
+	kern<<<dim3(nblocks), dim3(256), smem, s>>>(rec, fin, ftgt, dc, mc, clamp, chunks, w16, w16t, st, partials,
+	                                           pstride, overflow);
 	return hipGetLastError();
 }
 
 template <int H, int ENC>
-hipError_t grad_enc(const TrainArgs &a, uint32_t nb, hipStream_t s) {
+hipError_t grad_enc(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
 	if (a.loss == NRC_LOSS_L2)
 		return grad_launch<H, ENC, NRC_LOSS_L2, false>(a.records, nullptr, nullptr, a.d_count, a.max_count, a.clamp,
-		                                                a.w16, a.state, a.partials, a.pstride, &a.state->overflow, nb, s);
+		                                                chunks, a.w16, a.w16t, a.state, a.partials, a.pstride,
+		                                                &a.state->overflow, nb, s);
 	return grad_launch<H, ENC, NRC_LOSS_RELATIVE_L2_LUMINANCE, false>(a.records, nullptr, nullptr, a.d_count,
-	                                                                  a.max_count, a.clamp, a.w16, a.state, a.partials,
-	                                                                  a.pstride, &a.state->overflow, nb, s);
+	                                                                  a.max_count, a.clamp, chunks, a.w16, a.w16t,
+	                                                                  a.state, a.partials, a.pstride,
+	                                                                  &a.state->overflow, nb, s);
 }
 
-template <int H> hipError_t grad_h(const TrainArgs &a, uint32_t nb, hipStream_t s) {
+template <int H> hipError_t grad_h(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
 	switch (a.encoding) {
-	case NRC_ENCODING_TRI12: return grad_enc<H, NRC_ENCODING_TRI12>(a, nb, s);
-	case NRC_ENCODING_SINCOS6: return grad_enc<H, NRC_ENCODING_SINCOS6>(a, nb, s);
-	default: return grad_enc<H, NRC_ENCODING_UV_SINCOS16>(a, nb, s);
+	case NRC_ENCODING_TRI12: return grad_enc<H, NRC_ENCODING_TRI12>(a, chunks, nb, s);
+	case NRC_ENCODING_SINCOS6: return grad_enc<H, NRC_ENCODING_SINCOS6>(a, chunks, nb, s);
+	default: return grad_enc<H, NRC_ENCODING_UV_SINCOS16>(a, chunks, nb, s);
 	}
 }
 
@@ -582,26 +648,37 @@ hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t
 	return infer_launch<4, NRC_ENCODING_TRI12, true>(nullptr, inputs, nullptr, n, w16, out, 0, stream);
 }
 
-hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t *nparts, hipStream_t stream) {
+uint32_t fused_train_chunks(uint32_t n) {
+	const uint32_t chunks64 = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
+	const uint32_t target = (uint32_t)device_cu_count() / 2; // workgroups per step
+	uint32_t c = target ? (chunks64 + target - 1) / target : 1;
+	if (c < 1) c = 1;
+	if (c > kFusedMaxChunks) c = kFusedMaxChunks;
+	return c;
+}
+
+hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t chunks, uint32_t *nparts, hipStream_t stream) {
 	const uint32_t n = a.max_count < a.clamp ? a.max_count : a.clamp;
-	uint32_t nb = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
+	const uint32_t chunks64 = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
+	uint32_t nb = (chunks64 + chunks - 1) / chunks;
 	if (nb == 0) nb = 1;
 	*nparts = nb;
-	if (a.hidden == 5) return grad_h<5>(a, nb, stream);
-	return grad_h<4>(a, nb, stream);
+	if (a.hidden == 5) return grad_h<5>(a, chunks, nb, stream);
+	return grad_h<4>(a, chunks, nb, stream);
 }
 
 hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,
-                                     const _Float16 *w16, float *partials, uint32_t pstride, uint32_t *nparts,
-                                     hipStream_t stream) {
+                                     const _Float16 *w16, const _Float16 *w16t, float *partials, uint32_t pstride,
+                                     uint32_t *nparts, hipStream_t stream) {
 	uint32_t nb = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
 	if (nb == 0) nb = 1;
 	*nparts = nb;
 	if (hidden == 5)
-		return grad_launch<5, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, w16,
-		                                                             nullptr, partials, pstride, nullptr, nb, stream);
-	return grad_launch<4, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, w16, nullptr,
-	                                                             partials, pstride, nullptr, nb, stream);
+		return grad_launch<5, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, 1, w16,
+		                                                             w16t, nullptr, partials, pstride, nullptr, nb,
+		                                                             stream);
+	return grad_launch<4, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, 1, w16, w16t,
+	                                                             nullptr, partials, pstride, nullptr, nb, stream);
 }
 
 } // namespace nrc

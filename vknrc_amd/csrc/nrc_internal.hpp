@@ -11,8 +11,10 @@ namespace nrc {
 
 // Samples per chunk of the plain weight-gradient reduction.
 constexpr uint32_t kPlainChunk = 512;
-// Samples per workgroup of the fused gradient kernel.
+// Records per chunk of the fused gradient kernel (one 16-record N-tile per wave).
 constexpr uint32_t kFusedTrainSamples = 64;
+// Most chunks one fused-gradient workgroup walks.
+constexpr uint32_t kFusedMaxChunks = 16;
 
 struct TrainArgs {
 	int hidden;
@@ -24,6 +26,7 @@ struct TrainArgs {
 	uint32_t clamp; // train batch size
 	const float *w32;
 	const _Float16 *w16;
+	const _Float16 *w16t; // hidden layers transposed, [l][in][out]
 	DeviceState *state;
 	float *partials; // [nparts][pstride]
 	uint32_t pstride;
@@ -39,9 +42,10 @@ struct AdamArgs {
 	uint32_t max_count, clamp;
 	DeviceState *state;
 	nrc_optimizer_entry *entries;
-	_Float16 *w16, *use16;
+	_Float16 *w16, *w16t, *use16;
 	float *w32, *use32;
 	int write_use;
+	int hidden;
 	uint32_t weight_count;
 	AdamParams p;
 };
@@ -59,16 +63,20 @@ hipError_t launch_adam(const AdamArgs &a, hipStream_t stream);
 hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32_t pstride, uint32_t n, float *out,
                                   hipStream_t stream);
 hipError_t launch_init_state(DeviceState *st, float loss_scale, hipStream_t stream);
+// w16t[l][k][m] = w16[l][m][k] for the hidden layers
+hipError_t launch_transpose_hidden(int hidden, const _Float16 *w16, _Float16 *w16t, hipStream_t stream);
 
 // nrc_fused.hip
 hipError_t launch_fused_infer(int hidden, int encoding, const float *queries, const uint32_t *d_count,
                               uint32_t max_count, const _Float16 *w16, float *rgb, hipStream_t stream);
 hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *w16, float *out,
                                     hipStream_t stream);
-hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t *nparts, hipStream_t stream);
+// 64-record chunks per workgroup for a step of n records (about one workgroup per two CUs)
+uint32_t fused_train_chunks(uint32_t n);
+hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t chunks, uint32_t *nparts, hipStream_t stream);
 hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,
-                                     const _Float16 *w16, float *partials, uint32_t pstride, uint32_t *nparts,
-                                     hipStream_t stream);
+                                     const _Float16 *w16, const _Float16 *w16t, float *partials, uint32_t pstride,
+                                     uint32_t *nparts, hipStream_t stream);
 
 int device_cu_count();
 

@@ -282,6 +282,28 @@ def test_train_count_is_clamped_to_the_batch_size(o64):
     assert cosine(s["m"] / 0.1, G) > 0.99999
 
 
+@pytest.mark.parametrize("path", [v.PATH_PLAIN_F32, v.PATH_FUSED_F16])
+def test_partial_batch_from_device_count(o64, path):
+    """d_count well below max_count: workgroups past the count must contribute nothing,
+    including on a cache whose partial buffers hold an earlier full step's values."""
+    rng = np.random.default_rng(29)
+    st, w = make_cache(rng)
+    full = train_records(16384, 30)
+    st.train(dev(full), path=path)            # fills every partial slot
+    st.set_state(weight=w, state={"t": 0, "beta1_t": 1.0, "beta2_t": 1.0, "alpha_t": 1.0, "alpha_t_1": 0.0,
+                                  "loss_scale": 1.0})   # back to w with fresh moments and scalars
+    r = train_records(16384, 31)
+    c = torch.tensor([1000], dtype=torch.int32, device=DEV)
+    st.train(dev(r), count=c, path=path)
+    s = st.get_state()
+    assert s["state"]["last_count"] == 1000 and s["state"]["t"] == 1
+    is16 = path == v.PATH_FUSED_F16
+    G, _ = o64.gradient(f16(w) if is16 else w.astype(np.float64), 5, r[:1000], v.ENCODING_TRI12,
+                        v.LOSS_RELATIVE_L2_LUMINANCE, round_f16=is16)
+    g = s["m"].astype(np.float64) / (1.0 - np.float32(0.9))
+    assert cosine(g, G / 1000) > (GRAD_COSINE_MIN if is16 else 0.99999)
+
+
 def test_nonfinite_gradient_semantics():
     rng = np.random.default_rng(13)
     r = train_records(1024, 14)

@@ -23,6 +23,7 @@ struct nrc_cache {
 	DeviceState *state = nullptr;
 	float *partials = nullptr;
 	uint32_t part_cap = 0, pstride = 0;
+	int parity = 0; // state buffer the next train step reads (flipped per launched step)
 	// plain-path scratch (allocated on the first plain train step)
 	float *acts = nullptr, *deltas = nullptr, *sample_loss = nullptr;
 	uint32_t ldn = 0;
@@ -127,6 +128,7 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 	nrc_status s = upload_entries(c, e.data());
 	if (s != NRC_OK) return s;
 	NRC_HIP(nrc::launch_init_state(c->state, c->desc.loss_scale, nullptr), "init state");
+	c->parity = 0;
 	NRC_HIP(hipDeviceSynchronize(), "reset");
 	c->desc.seed = seed;
 	return NRC_OK;
@@ -304,6 +306,7 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	a.w16 = c->w16;
 	a.w16t = c->w16t;
 	a.state = c->state;
+	a.parity = c->parity;
 	a.partials = c->partials;
 	a.pstride = c->pstride;
 	// partial-gradient slots the launch will write, checked before anything runs
@@ -334,6 +337,7 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	o.max_count = max_count;
 	o.clamp = c->desc.train_batch_size;
 	o.state = c->state;
+	o.parity = c->parity;
 	o.entries = c->entries;
 	o.w16 = c->w16;
 	o.w16t = c->w16t;
@@ -346,6 +350,7 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	o.p = adam_params(c);
 	e = nrc::launch_adam(o, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
+	c->parity ^= 1; // the optimizer wrote the other state buffer
 	return NRC_OK;
 }
 
@@ -370,7 +375,9 @@ nrc_status nrc_get_state(nrc_cache *c, nrc_optimizer_entry *entries, nrc_optimiz
 	NRC_HIP(hipDeviceSynchronize(), "nrc_get_state");
 	if (entries)
 		NRC_HIP(hipMemcpy(entries, c->entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyDeviceToHost), "get entries");
-	if (state) NRC_HIP(hipMemcpy(state, &c->state->pub, sizeof(nrc_optimizer_state), hipMemcpyDeviceToHost), "get state");
+	if (state)
+		NRC_HIP(hipMemcpy(state, &c->state->pub[c->parity], sizeof(nrc_optimizer_state), hipMemcpyDeviceToHost),
+		        "get state");
 	if (w16) NRC_HIP(hipMemcpy(w16, c->w16, 2 * c->wc, hipMemcpyDeviceToHost), "get weights");
 	if (use16) NRC_HIP(hipMemcpy(use16, c->use16, 2 * c->wc, hipMemcpyDeviceToHost), "get use weights");
 	return NRC_OK;
@@ -386,7 +393,8 @@ nrc_status nrc_set_state(nrc_cache *c, const nrc_optimizer_entry *entries, const
 	}
 	if (state) {
 		DeviceState ds{};
-		ds.pub = *state;
+		ds.pub[0] = *state;
+		ds.pub[1] = *state;
 		NRC_HIP(hipMemcpy(c->state, &ds, sizeof(ds), hipMemcpyHostToDevice), "set state");
 	}
 	NRC_HIP(hipDeviceSynchronize(), "nrc_set_state");

@@ -15,14 +15,14 @@ constexpr int kTrainFloats = 17;
 
 __host__ __device__ constexpr int weight_count(int hidden) { return hidden * kWidth * kWidth + kOut * kWidth; }
 
-// Device-resident optimizer state: the public part plus the cross-block
-// bookkeeping of the optimizer kernel.
+// Device-resident optimizer state, double-buffered by a parity the host flips
+// on every launched train step: step p reads pub[p] and writes pub[p ^ 1], and
+// its gradient kernels flag non-finite partials in overflow[p] (the optimizer
+// clears overflow[p ^ 1] for the next step).  No block ever reads a buffer
+// another block of the same launch writes, so no cross-block counter is needed.
 struct DeviceState {
-	nrc_optimizer_state pub;
-	uint32_t done_blocks; // blocks of the running optimizer launch that finished
-	uint32_t overflow;    // set by gradient kernels when a partial is non-finite
-	float loss_sum;       // summed loss of the running optimizer step
-	uint32_t pad;
+	nrc_optimizer_state pub[2];
+	uint32_t overflow[2];
 };
 
 struct AdamParams {

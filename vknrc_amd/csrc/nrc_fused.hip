@@ -17,6 +17,7 @@
 // fp16 weights of layer l, row-major [out][in], at l*4096; the 3x64 output
 // layer at H*4096 (padded to 16 rows with zeros inside the MFMA, NN_nv.glsl:77-82).
 #include <cstdlib>
+#include <type_traits>
 
 #include "nrc_internal.hpp"
 
@@ -64,12 +65,27 @@ __device__ __forceinline__ void wave_lds_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// Row-major [rows][64] fp16 weights -> padded LDS rows.
-__device__ __forceinline__ void stage_rows(_Float16 *sW, const _Float16 *__restrict__ w16, int rows) {
-	const int chunks = rows * 8;
-	for (int ci = threadIdx.x; ci < chunks; ci += blockDim.x) {
-		const int row = ci >> 3, c8 = ci & 7;
-		*reinterpret_cast<uint4 *>(sW + row * kLdw + c8 * 8) = reinterpret_cast<const uint4 *>(w16)[ci];
+// Row-major [ROWS][64] fp16 weights -> padded LDS rows, by a 256-thread block.
+// Loads go out four at a time before their LDS stores, so the block waits for
+// one L2 round trip per four 16-byte chunks per thread instead of one per chunk.
+template <int ROWS> __device__ __forceinline__ void stage_rows(_Float16 *sW, const _Float16 *__restrict__ w16) {
+	constexpr int kChunks = ROWS * 8;
+	const uint4 *src = reinterpret_cast<const uint4 *>(w16);
+	auto put = [&](int ci, uint4 v) __attribute__((always_inline)) {
+		*reinterpret_cast<uint4 *>(sW + (ci >> 3) * kLdw + (ci & 7) * 8) = v;
+	};
+#pragma unroll
+	for (int b = 0; b < kChunks; b += 1024) {
+		const int c0 = b + threadIdx.x, c1 = c0 + 256, c2 = c0 + 512, c3 = c0 + 768;
+		uint4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
+		if (c0 < kChunks) v0 = src[c0];
+		if (c1 < kChunks) v1 = src[c1];
+		if (c2 < kChunks) v2 = src[c2];
+		if (c3 < kChunks) v3 = src[c3];
+		if (c0 < kChunks) put(c0, v0);
+		if (c1 < kChunks) put(c1, v1);
+		if (c2 < kChunks) put(c2, v2);
+		if (c3 < kChunks) put(c3, v3);
 	}
 }
 
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 	__shared__ __attribute__((aligned(16))) _Float16 sW[kRows * kLdw];
 	__shared__ __attribute__((aligned(16))) half8_t sX[4][8 * 64];
 
-	stage_rows(sW, w16, kRows);
+	stage_rows<kRows>(sW, w16);
 	__syncthreads();
 
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -297,103 +313,204 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 
 // ================================================================ gradient
 //
-// One workgroup = 4 waves; it walks `chunks` consecutive 64-record chunks (wave
-// w owns N-tile w of each chunk) and accumulates its part of dW in VGPRs.  LDS
-// holds the fp16 weights row-major and transposed (the transposed copy is
-// written by the optimizer, so staging is a straight copy), every layer's
-// activations feature-major [feature][record] -- the K = records operand
-// layout of the dW MFMAs -- and the current layer's delta.
+// One workgroup = 8 waves (2 per SIMD) walking `chunks` consecutive 128-record
+// chunks; wave w owns N-tile w (records 16w..16w+15) of each chunk.  LDS holds
+// the fp16 weights row-major and transposed (the optimizer writes the
+// transposed copy, so staging is a straight copy) and, double-buffered, the
+// current layer's activations and deltas feature-major [feature][record] -- the
+// K = records operand layout of the dW MFMAs.  One barrier per layer.
 //   forward   : per wave, B operands in VGPRs, A operands from LDS
-//   dW_l^T    : wave w owns output rows 16w..16w+15 of every layer; computing
-//               the transposed tile (A = a_l, B = delta_l) leaves 4 consecutive
-//               input columns per lane, so partials leave as float4 stores
+//   dW_l^T    : wave w owns the 16x16 tiles (out rows 16(w&3).., in columns
+//               16(2(w>>2)+j)..), K = the 128 records; the transposed tile
+//               (A = a_l, B = delta_l) leaves 4 consecutive input columns per
+//               lane, so partials leave as float4 stores
 //   delta_l-1 : per wave, (W_l^T delta_l) . 1[a_l > 0] with A from the
 //               transposed copy and the ReLU mask from the saved B operands
 // The workgroup writes its partial dW (FP32) and loss sum once; the optimizer
 // kernel reduces the partials in a fixed order.
+constexpr int kGradWaves = 8;
+constexpr int kGradRecords = 16 * kGradWaves; // records per chunk
+constexpr int kGLdw = 68;                     // halves per LDS weight row (64 + 4 padding)
+constexpr int kGLdn = kGradRecords + 8;       // halves per LDS activation row
+
 template <int H> struct GradSmem {
-	_Float16 w[(H * kWidth + kOut) * kLdw];
-	_Float16 wt[H * kWidth * kLdw];
-	_Float16 act[(H + 1) * kWidth * kLdn];
-	_Float16 delta[kWidth * kLdn];
-	float tgt[64 * 3];
-	float loss[64];
+	_Float16 w[(H * kWidth + kOut) * kGLdw];
+	_Float16 wt[H * kWidth * kGLdw];
+	_Float16 act[2][kWidth * kGLdn];   // a_l feature-major [feature][record], double-buffered
+	_Float16 delta[2][kWidth * kGLdn]; // delta_l feature-major [feature][record], double-buffered
+	float tgt[kGradRecords * 3];
+	float loss[kGradRecords];
 };
+
+// Row-major [ROWS][64] fp16 weights -> LDS rows of kGLdw halves, by a 512-thread
+// block: loads go out in batches before their LDS stores (8-byte stores, since
+// the 136-byte rows are 8-byte aligned).
+template <int ROWS> __device__ __forceinline__ void stage_rows_grad(_Float16 *sW, const _Float16 *__restrict__ w16) {
+	constexpr int kChunks = ROWS * 8;
+	const uint4 *src = reinterpret_cast<const uint4 *>(w16);
+	auto put = [&](int ci, uint4 v) __attribute__((always_inline)) {
+		uint2 *d = reinterpret_cast<uint2 *>(sW + (ci >> 3) * kGLdw + (ci & 7) * 8);
+		d[0] = make_uint2(v.x, v.y);
+		d[1] = make_uint2(v.z, v.w);
+	};
+#pragma unroll
+	for (int b = 0; b < kChunks; b += 2048) {
+		const int c0 = b + threadIdx.x, c1 = c0 + 512, c2 = c0 + 1024, c3 = c0 + 1536;
+		uint4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
+		if (c0 < kChunks) v0 = src[c0];
+		if (c1 < kChunks) v1 = src[c1];
+		if (c2 < kChunks) v2 = src[c2];
+		if (c3 < kChunks) v3 = src[c3];
+		if (c0 < kChunks) put(c0, v0);
+		if (c1 < kChunks) put(c1, v1);
+		if (c2 < kChunks) put(c2, v2);
+		if (c3 < kChunks) put(c3, v3);
+	}
+}
+
+__device__ __forceinline__ half8_t load_afrag_g(const _Float16 *sW, int row, int c, int g) {
+	const half4_t lo = *reinterpret_cast<const half4_t *>(sW + row * kGLdw + 32 * c + 4 * g);
+	const half4_t hi = *reinterpret_cast<const half4_t *>(sW + row * kGLdw + 32 * c + 16 + 4 * g);
+	half8_t a;
+	a[0] = lo[0]; a[1] = lo[1]; a[2] = lo[2]; a[3] = lo[3];
+	a[4] = hi[0]; a[5] = hi[1]; a[6] = hi[2]; a[7] = hi[3];
+	return a;
+}
 
 // feature index of element j of a B operand held by lane group g for K-chunk c
 __device__ __forceinline__ int bfeat(int c, int g, int j) { return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4); }
 
-template <int H, int ENC, int LOSS, bool F16IN>
-__global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict__ records,
+// writes this lane's 16 values of a B operand (record column col) feature-major
+__device__ __forceinline__ void store_bcol(_Float16 *dst, const half8_t *b, int g, int col) {
+#pragma unroll
+	for (int c = 0; c < 2; ++c)
+#pragma unroll
+		for (int j = 0; j < 8; ++j) dst[bfeat(c, g, j) * kGLdn + col] = b[c][j];
+}
+
+template <int H, int ENC, int LOSS, bool F16IN, bool MULTI>
+__global__ __launch_bounds__(512) void fused_grad_kernel(const float *__restrict__ records,
                                                          const _Float16 *__restrict__ fin,
                                                          const float *__restrict__ ftgt,
                                                          const uint32_t *__restrict__ d_count, uint32_t max_count,
                                                          uint32_t clamp, uint32_t chunks,
                                                          const _Float16 *__restrict__ w16,
                                                          const _Float16 *__restrict__ w16t,
-                                                         const DeviceState *__restrict__ st,
+                                                         const DeviceState *__restrict__ st, int parity,
                                                          float *__restrict__ partials, uint32_t pstride,
                                                          uint32_t *__restrict__ overflow) {
 	constexpr int WC = weight_count(H);
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
 	GradSmem<H> &S = *reinterpret_cast<GradSmem<H> *>(smem_raw);
 
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int g = lane >> 4, r = lane & 15;
+	const int tid = threadIdx.x, wave = tid >> 6;
 	const uint32_t count = F16IN ? max_count : effective_count(d_count, max_count, clamp);
-	const float scale = (F16IN || st == nullptr) ? 1.0f : st->pub.loss_scale;
-	const int col = 16 * wave + r; // this lane's record column inside a chunk
+	const float scale = (F16IN || st == nullptr) ? 1.0f : st->pub[parity].loss_scale;
+	const int enc_q = wave & 3;                       // feature block this wave encodes
+	const int mt_w = wave & 3, nt0 = 2 * (wave >> 2); // dW tiles of this wave: (mt_w, nt0), (mt_w, nt0 + 1)
+	const uint32_t last = count ? count - 1 : 0;
+	int lane = tid & 63;
+	int g = lane >> 4, r = lane & 15;
+	int col = 16 * wave + r;               // this lane's record column inside a chunk
+	int enc_rec = 64 * (wave >> 2) + lane; // record this lane encodes
 
-	stage_rows(S.w, w16, H * kWidth + kOut);
-	stage_rows(S.wt, w16t, H * kWidth);
+	// this lane's record to encode in chunk ch: the index is clamped into [0, count)
+	// so the loads are unconditional; lanes past the count are zeroed at encode time
+	float rec[kTrainFloats];
+	auto load_rec = [&](uint32_t ch) __attribute__((always_inline)) {
+		uint32_t n = (blockIdx.x * chunks + ch) * (uint32_t)kGradRecords + enc_rec;
+		n = n < last ? n : last;
+		if constexpr (!F16IN) {
+			const float *rp = records + (size_t)n * kTrainFloats;
+#pragma unroll
+			for (int i = 0; i < kTrainFloats; ++i) rec[i] = rp[i];
+		}
+	};
+	load_rec(0); // in flight while the weights are staged
 
-	float4_t accW[H][4]; // dW_l^T tiles: rows = input columns 16nt + 4g + i, column = output row 16w + r
-	float4_t accO = (float4_t)0.0f;
+	stage_rows_grad<H * kWidth + kOut>(S.w, w16);
+	stage_rows_grad<H * kWidth>(S.wt, w16t);
+
+	float4_t accW[MULTI ? H : 1][2]; // multi-chunk accumulators of this wave's two dW tiles per layer
+	float4_t accO = (float4_t)0.0f;   // dW_H^T tile (waves 0..3)
 #pragma unroll
-	for (int l = 0; l < H; ++l)
-#pragma unroll
-		for (int nt = 0; nt < 4; ++nt) accW[l][nt] = (float4_t)0.0f;
+	for (int l = 0; l < (MULTI ? H : 1); ++l) accW[l][0] = accW[l][1] = (float4_t)0.0f;
 	float loss_sum = 0.0f;
+	float *part = partials + (size_t)blockIdx.x * pstride;
+	bool bad = false;
+	auto store_tile = [&](int l, int nt, float4_t v4) __attribute__((always_inline)) {
+		*reinterpret_cast<float4_t *>(part + l * kWidth * kWidth + (16 * mt_w + r) * kWidth + 16 * nt + 4 * g) = v4;
+#pragma unroll
+		for (int i = 0; i < 4; ++i) bad |= !isfinite(v4[i]);
+	};
 
 	for (uint32_t ch = 0; ch < chunks; ++ch) {
-		const uint32_t base = (blockIdx.x * chunks + ch) * 64u;
-		if (base >= count) break; // uniform over the workgroup
-		__syncthreads();          // previous chunk's LDS readers are done
-		// ---- encode: wave w computes features 16w..16w+15 of all 64 records
+		const uint32_t base = (blockIdx.x * chunks + ch) * (uint32_t)kGradRecords;
+		if (base >= count) { // uniform over the workgroup
+			if constexpr (!MULTI) {
+				// a chunk wholly past the count still owns its partial slot: zeros
+#pragma unroll
+				for (int l = 0; l < H; ++l) {
+					store_tile(l, nt0, (float4_t)0.0f);
+					store_tile(l, nt0 + 1, (float4_t)0.0f);
+				}
+			}
+			break;
+		}
+		// recompute the lane-derived indices each chunk from an opaque copy of the
+		// lane id, so the many LDS addresses built from them are not hoisted out of
+		// the loop and kept live through the encoder
+		asm volatile("" : "+v"(lane));
+		g = lane >> 4;
+		r = lane & 15;
+		col = 16 * wave + r;
+		enc_rec = 64 * (wave >> 2) + lane;
+		__syncthreads(); // staging done / previous chunk's LDS readers done
+		// ---- encode: wave w computes features 16(w&3).. of records 64(w>>2) + lane into act[0]
 		{
-			const uint32_t n = base + lane;
+			const uint32_t n = base + enc_rec;
 			const bool valid = n < count;
-			const uint32_t ns = valid ? n : 0u;
-			_Float16 *a0 = S.act;
+			_Float16 *a0 = S.act[0];
 			if constexpr (F16IN) {
+				const uint32_t ns = n < last ? n : last;
 #pragma unroll
 				for (int j = 0; j < 16; ++j)
-					a0[(16 * wave + j) * kLdn + lane] = valid ? fin[(size_t)ns * kWidth + 16 * wave + j] : (_Float16)0.0f;
-				if (wave == 0)
+					a0[(16 * enc_q + j) * kGLdn + enc_rec] =
+					    valid ? fin[(size_t)ns * kWidth + 16 * enc_q + j] : (_Float16)0.0f;
+				if (enc_q == 0)
 #pragma unroll
-					for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? ftgt[(size_t)ns * 3 + c] : 0.0f;
+					for (int c = 0; c < 3; ++c) S.tgt[enc_rec * 3 + c] = valid ? ftgt[(size_t)ns * 3 + c] : 0.0f;
 			} else {
-				float rec[kQueryFloats];
-				const float *rp = records + (size_t)ns * kTrainFloats;
+				// one branch per feature block, each writing its own 16 features (no
+				// 16-value merge across the branches)
+				auto put = [&](auto qc) __attribute__((always_inline)) {
+					constexpr int q = decltype(qc)::value;
+					float f[16];
+					encode_block<ENC>(rec, q, f);
 #pragma unroll
-				for (int i = 0; i < kQueryFloats; ++i) rec[i] = valid ? rp[i] : 0.0f;
-				float f[16];
-				encode_block<ENC>(rec, wave, f);
+					for (int j = 0; j < 16; ++j)
+						a0[(16 * q + j) * kGLdn + enc_rec] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+				};
+				switch (enc_q) {
+				case 0: put(std::integral_constant<int, 0>{}); break;
+				case 1: put(std::integral_constant<int, 1>{}); break;
+				case 2: put(std::integral_constant<int, 2>{}); break;
+				default: put(std::integral_constant<int, 3>{}); break;
+				}
+				if (enc_q == 0)
 #pragma unroll
-				for (int j = 0; j < 16; ++j) a0[(16 * wave + j) * kLdn + lane] = valid ? (_Float16)f[j] : (_Float16)0.0f;
-				if (wave == 0)
-#pragma unroll
-					for (int c = 0; c < 3; ++c) S.tgt[lane * 3 + c] = valid ? rp[14 + c] : 0.0f;
+					for (int c = 0; c < 3; ++c) S.tgt[enc_rec * 3 + c] = valid ? rec[14 + c] : 0.0f;
 			}
 		}
+		if (MULTI && ch + 1 < chunks) load_rec(ch + 1); // next chunk's records in flight during this one
 		__syncthreads();
 
-		// ---- forward for this wave's 16 records
-		half8_t Bs[H + 1][2]; // fp16 activations a_0..a_H as B operands
+		// ---- forward for this wave's 16 records; activations stay in VGPRs as B operands
+		half8_t Bs[H + 1][2];
 #pragma unroll
 		for (int c = 0; c < 2; ++c)
 #pragma unroll
-			for (int j = 0; j < 8; ++j) Bs[0][c][j] = S.act[bfeat(c, g, j) * kLdn + col];
+			for (int j = 0; j < 8; ++j) Bs[0][c][j] = S.act[0][bfeat(c, g, j) * kGLdn + col];
 #pragma unroll
 		for (int l = 0; l < H; ++l) {
 			float4_t acc[4];
@@ -403,19 +520,15 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt)
-					acc[mt] = mfma(load_afrag(S.w, l * kWidth + mt * 16 + r, c, g), Bs[l][c], acc[mt]);
+					acc[mt] = mfma(load_afrag_g(S.w, l * kWidth + mt * 16 + r, c, g), Bs[l][c], acc[mt]);
 #pragma unroll
 			for (int c = 0; c < 2; ++c) Bs[l + 1][c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
-			_Float16 *al = S.act + (l + 1) * kWidth * kLdn;
-#pragma unroll
-			for (int c = 0; c < 2; ++c)
-#pragma unroll
-				for (int j = 0; j < 8; ++j) al[bfeat(c, g, j) * kLdn + col] = Bs[l + 1][c][j];
+			__builtin_amdgcn_sched_barrier(0);
 		}
 		float4_t o = (float4_t)0.0f;
 #pragma unroll
 		for (int c = 0; c < 2; ++c) {
-			half8_t a = load_afrag(S.w, H * kWidth + (r < kOut ? r : 0), c, g);
+			half8_t a = load_afrag_g(S.w, H * kWidth + (r < kOut ? r : 0), c, g);
 			if (r >= kOut) a = (half8_t)0;
 			o = mfma(a, Bs[H][c], o);
 		}
@@ -435,20 +548,25 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 				Bg[1] = (_Float16)go[1];
 				Bg[2] = (_Float16)go[2];
 			}
-			// delta rows 0..15 for record col: rows 0..2 = g_o, the rest 0 (padded output rows)
+			// delta[1] rows 0..15 for record col: rows 0..2 = g_o, the rest 0 (padded output rows);
+			// act[1] = a_H for the dW_H MFMAs
 #pragma unroll
-			for (int i = 0; i < 4; ++i) S.delta[(4 * g + i) * kLdn + col] = (g == 0 && i < 3) ? Bg[i] : (_Float16)0.0f;
+			for (int i = 0; i < 4; ++i) S.delta[1][(4 * g + i) * kGLdn + col] = (g == 0 && i < 3) ? Bg[i] : (_Float16)0.0f;
+			store_bcol(S.act[1], Bs[H], g, col);
 		}
 		__syncthreads();
 		if (tid == 0)
-			for (int i = 0; i < 64; ++i) loss_sum += S.loss[i];
+			for (int i = 0; i < kGradRecords; ++i) loss_sum += S.loss[i];
 
-		// ---- dW_H^T: rows = input columns 16w + 4g + i, column = output row r (< 3 valid)
+		// ---- dW_H^T (waves 0..3): rows = input columns 16w + 4g + i, column = output row r (< 3 valid)
+		if (wave < 4) {
 #pragma unroll
-		for (int c = 0; c < 2; ++c) {
-			const half8_t a = *reinterpret_cast<const half8_t *>(S.act + (H * kWidth + 16 * wave + r) * kLdn + 32 * c + 8 * g);
-			const half8_t b = *reinterpret_cast<const half8_t *>(S.delta + r * kLdn + 32 * c + 8 * g);
-			accO = mfma(a, b, accO);
+			for (int c = 0; c < kGradRecords / 32; ++c) {
+				const half8_t a =
+				    *reinterpret_cast<const half8_t *>(S.act[1] + (16 * wave + r) * kGLdn + 32 * c + 8 * g);
+				const half8_t b = *reinterpret_cast<const half8_t *>(S.delta[1] + r * kGLdn + 32 * c + 8 * g);
+				accO = mfma(a, b, accO);
+			}
 		}
 		// ---- delta_{H-1} = (W_H^T g_o) . 1[a_H > 0] for this wave's records
 		half8_t Bd[2];
@@ -459,9 +577,9 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 				half8_t a = (half8_t)0;
 				if (g == 0) {
 					const int m = 16 * mt + r;
-					a[0] = S.w[(H * kWidth + 0) * kLdw + m];
-					a[1] = S.w[(H * kWidth + 1) * kLdw + m];
-					a[2] = S.w[(H * kWidth + 2) * kLdw + m];
+					a[0] = S.w[(H * kWidth + 0) * kGLdw + m];
+					a[1] = S.w[(H * kWidth + 1) * kGLdw + m];
+					a[2] = S.w[(H * kWidth + 2) * kGLdw + m];
 				}
 				acc[mt] = mfma(a, Bg, (float4_t)0.0f);
 			}
@@ -473,28 +591,40 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 					Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[H][c][4 + i]);
 				}
 		}
-		__syncthreads(); // every wave has finished reading the g_o rows of delta
-#pragma unroll
-		for (int c = 0; c < 2; ++c)
-#pragma unroll
-			for (int j = 0; j < 8; ++j) S.delta[bfeat(c, g, j) * kLdn + col] = Bd[c][j];
+		// buffer 0 is free (its last readers passed the barrier above): delta_{H-1}, a_{H-1}
+		store_bcol(S.delta[0], Bd, g, col);
+		store_bcol(S.act[0], Bs[H - 1], g, col);
 		__syncthreads();
 
-		// ---- layers H-1 .. 0
+		// ---- layers H-1 .. 0: buffer p holds delta_l and a_l
 #pragma unroll
 		for (int l = H - 1; l >= 0; --l) {
-			// dW_l^T tiles for output rows 16w .. 16w+15, all 64 input columns
-#pragma unroll
-			for (int c = 0; c < 2; ++c) {
-				const half8_t b = *reinterpret_cast<const half8_t *>(S.delta + (16 * wave + r) * kLdn + 32 * c + 8 * g);
-#pragma unroll
-				for (int nt = 0; nt < 4; ++nt) {
-					const half8_t a =
-					    *reinterpret_cast<const half8_t *>(S.act + (l * kWidth + 16 * nt + r) * kLdn + 32 * c + 8 * g);
-					accW[l][nt] = mfma(a, b, accW[l][nt]);
+			const int p = (H - 1 - l) & 1;
+			// this wave's two dW_l^T tiles over the 128 records
+			{
+				float4_t t0 = MULTI ? accW[MULTI ? l : 0][0] : (float4_t)0.0f;
+				float4_t t1 = MULTI ? accW[MULTI ? l : 0][1] : (float4_t)0.0f;
+#pragma unroll 2
+				for (int c = 0; c < kGradRecords / 32; ++c) {
+					const half8_t b =
+					    *reinterpret_cast<const half8_t *>(S.delta[p] + (16 * mt_w + r) * kGLdn + 32 * c + 8 * g);
+					const half8_t a0 =
+					    *reinterpret_cast<const half8_t *>(S.act[p] + (16 * nt0 + r) * kGLdn + 32 * c + 8 * g);
+					const half8_t a1 =
+					    *reinterpret_cast<const half8_t *>(S.act[p] + (16 * nt0 + 16 + r) * kGLdn + 32 * c + 8 * g);
+					t0 = mfma(a0, b, t0);
+					t1 = mfma(a1, b, t1);
+				}
+				if constexpr (MULTI) {
+					accW[MULTI ? l : 0][0] = t0;
+					accW[MULTI ? l : 0][1] = t1;
+				} else {
+					store_tile(l, nt0, t0);
+					store_tile(l, nt0 + 1, t1);
 				}
 			}
 			if (l == 0) break;
+			__builtin_amdgcn_sched_barrier(0);
 			// delta_{l-1} = (W_l^T delta_l) . 1[a_l > 0]
 			{
 				float4_t acc[4];
@@ -504,7 +634,7 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 				for (int c = 0; c < 2; ++c)
 #pragma unroll
 					for (int mt = 0; mt < 4; ++mt)
-						acc[mt] = mfma(load_afrag(S.wt, l * kWidth + mt * 16 + r, c, g), Bd[c], acc[mt]);
+						acc[mt] = mfma(load_afrag_g(S.wt, l * kWidth + mt * 16 + r, c, g), Bd[c], acc[mt]);
 #pragma unroll
 				for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -513,28 +643,24 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 						Bd[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)Bs[l][c][4 + i]);
 					}
 			}
-			__syncthreads(); // dW_l reads of delta are done
-#pragma unroll
-			for (int c = 0; c < 2; ++c)
-#pragma unroll
-				for (int j = 0; j < 8; ++j) S.delta[bfeat(c, g, j) * kLdn + col] = Bd[c][j];
+			// the other buffer's last readers passed the previous barrier
+			__builtin_amdgcn_sched_barrier(0);
+			store_bcol(S.delta[p ^ 1], Bd, g, col);
+			store_bcol(S.act[p ^ 1], Bs[l - 1], g, col);
 			__syncthreads();
+			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
 
 	// ---- partial dW of this workgroup (float4 per lane and tile) and its loss sum
-	float *part = partials + (size_t)blockIdx.x * pstride;
-	bool bad = false;
+	if constexpr (MULTI) {
 #pragma unroll
-	for (int l = 0; l < H; ++l)
-#pragma unroll
-		for (int nt = 0; nt < 4; ++nt) {
-			const float4_t v4 = accW[l][nt];
-			*reinterpret_cast<float4_t *>(part + l * kWidth * kWidth + (16 * wave + r) * kWidth + 16 * nt + 4 * g) = v4;
-#pragma unroll
-			for (int i = 0; i < 4; ++i) bad |= !isfinite(v4[i]);
+		for (int l = 0; l < H; ++l) {
+			store_tile(l, nt0, accW[MULTI ? l : 0][0]);
+			store_tile(l, nt0 + 1, accW[MULTI ? l : 0][1]);
 		}
-	if (r < kOut) {
+	}
+	if (wave < 4 && r < kOut) {
 		*reinterpret_cast<float4_t *>(part + H * kWidth * kWidth + r * kWidth + 16 * wave + 4 * g) = accO;
 #pragma unroll
 		for (int i = 0; i < 4; ++i) bad |= !isfinite(accO[i]);
@@ -543,12 +669,13 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(const float *__restrict
 	if (bad && overflow) atomicOr(overflow, 1u);
 }
 
-// N-tiles each wave runs through the layers together (1 or 2); VKNRC_INFER_TILES overrides.
+// N-tiles each wave runs through the layers together (1 or 2; both measure the
+// same, and 1 needs no register spill); VKNRC_INFER_TILES overrides.
 int infer_tiles() {
 	static int t = 0;
 	if (t == 0) {
 		const char *e = getenv("VKNRC_INFER_TILES");
-		t = (e && atoi(e) == 1) ? 1 : 2;
+		t = (e && atoi(e) == 2) ? 2 : 1;
 	}
 	return t;
 }
@@ -571,19 +698,19 @@ hipError_t infer_launch(const float *q, const _Float16 *fin, const uint32_t *dc,
 template <int H, int ENC, int LOSS, bool F16IN>
 hipError_t grad_launch(const float *rec, const _Float16 *fin, const float *ftgt, const uint32_t *dc, uint32_t mc,
                        uint32_t clamp, uint32_t chunks, const _Float16 *w16, const _Float16 *w16t,
-                       const DeviceState *st, float *partials, uint32_t pstride, uint32_t *overflow, uint32_t nblocks,
-                       hipStream_t s) {
+                       const DeviceState *st, int parity, float *partials, uint32_t pstride, uint32_t *overflow,
+                       uint32_t nblocks, hipStream_t s) {
 	const size_t smem = sizeof(GradSmem<H>);
-	auto kern = fused_grad_kernel<H, ENC, LOSS, F16IN>;
-	static bool attr_set = false;
-	if (!attr_set) {
+	auto kern = chunks > 1 ? fused_grad_kernel<H, ENC, LOSS, F16IN, true> : fused_grad_kernel<H, ENC, LOSS, F16IN, false>;
+	static bool attr_set[2] = {false, false};
+	if (!attr_set[chunks > 1]) {
 		hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
 		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
 		if (e != hipSuccess) return e;
-		attr_set = true;
+		attr_set[chunks > 1] = true;
 	}
-	kern<<<dim3(nblocks), dim3(256), smem, s>>>(rec, fin, ftgt, dc, mc, clamp, chunks, w16, w16t, st, partials,
-	                                           pstride, overflow);
+	kern<<<dim3(nblocks), dim3(512), smem, s>>>(rec, fin, ftgt, dc, mc, clamp, chunks, w16, w16t, st, parity,
+	                                           partials, pstride, overflow);
 	return hipGetLastError();
 }
 
@@ -591,12 +718,12 @@ template <int H, int ENC>
 hipError_t grad_enc(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
 	if (a.loss == NRC_LOSS_L2)
 		return grad_launch<H, ENC, NRC_LOSS_L2, false>(a.records, nullptr, nullptr, a.d_count, a.max_count, a.clamp,
-		                                                chunks, a.w16, a.w16t, a.state, a.partials, a.pstride,
-		                                                &a.state->overflow, nb, s);
+		                                                chunks, a.w16, a.w16t, a.state, a.parity, a.partials,
+		                                                a.pstride, &a.state->overflow[a.parity], nb, s);
 	return grad_launch<H, ENC, NRC_LOSS_RELATIVE_L2_LUMINANCE, false>(a.records, nullptr, nullptr, a.d_count,
 	                                                                  a.max_count, a.clamp, chunks, a.w16, a.w16t,
-	                                                                  a.state, a.partials, a.pstride,
-	                                                                  &a.state->overflow, nb, s);
+	                                                                  a.state, a.parity, a.partials, a.pstride,
+	                                                                  &a.state->overflow[a.parity], nb, s);
 }
 
 template <int H> hipError_t grad_h(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
@@ -649,9 +776,15 @@ hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t
 }
 
 uint32_t fused_train_chunks(uint32_t n) {
-	const uint32_t chunks64 = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
-	const uint32_t target = (uint32_t)device_cu_count() / 2; // workgroups per step
-	uint32_t c = target ? (chunks64 + target - 1) / target : 1;
+	static int forced = -1;
+	if (forced < 0) {
+		const char *e = getenv("VKNRC_TRAIN_CHUNKS");
+		forced = e ? atoi(e) : 0;
+	}
+	if (forced > 0) return (uint32_t)(forced < (int)kFusedMaxChunks ? forced : (int)kFusedMaxChunks);
+	const uint32_t nchunks = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
+	const uint32_t target = (uint32_t)device_cu_count(); // workgroups per step: at most one per CU
+	uint32_t c = target ? (nchunks + target - 1) / target : 1;
 	if (c < 1) c = 1;
 	if (c > kFusedMaxChunks) c = kFusedMaxChunks;
 	return c;
@@ -659,8 +792,8 @@ uint32_t fused_train_chunks(uint32_t n) {
 
 hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t chunks, uint32_t *nparts, hipStream_t stream) {
 	const uint32_t n = a.max_count < a.clamp ? a.max_count : a.clamp;
-	const uint32_t chunks64 = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
-	uint32_t nb = (chunks64 + chunks - 1) / chunks;
+	const uint32_t nchunks = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
+	uint32_t nb = (nchunks + chunks - 1) / chunks;
 	if (nb == 0) nb = 1;
 	*nparts = nb;
 	if (a.hidden == 5) return grad_h<5>(a, chunks, nb, stream);
@@ -675,10 +808,10 @@ hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const f
 	*nparts = nb;
 	if (hidden == 5)
 		return grad_launch<5, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, 1, w16,
-		                                                             w16t, nullptr, partials, pstride, nullptr, nb,
+		                                                             w16t, nullptr, 0, partials, pstride, nullptr, nb,
 		                                                             stream);
 	return grad_launch<4, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, 1, w16, w16t,
-	                                                             nullptr, partials, pstride, nullptr, nb, stream);
+	                                                             nullptr, 0, partials, pstride, nullptr, nb, stream);
 }
 
 } // namespace nrc

@@ -11,8 +11,8 @@ namespace nrc {
 
 // Samples per chunk of the plain weight-gradient reduction.
 constexpr uint32_t kPlainChunk = 512;
-// Records per chunk of the fused gradient kernel (one 16-record N-tile per wave).
-constexpr uint32_t kFusedTrainSamples = 64;
+// Records per chunk of the fused gradient kernel (8 waves, one 16-record N-tile each).
+constexpr uint32_t kFusedTrainSamples = 128;
 // Most chunks one fused-gradient workgroup walks.
 constexpr uint32_t kFusedMaxChunks = 16;
 
@@ -28,6 +28,7 @@ struct TrainArgs {
 	const _Float16 *w16;
 	const _Float16 *w16t; // hidden layers transposed, [l][in][out]
 	DeviceState *state;
+	int parity;      // state buffer this step reads
 	float *partials; // [nparts][pstride]
 	uint32_t pstride;
 	// plain-path scratch, feature-major [(hidden+1)][64][ldn]
@@ -41,6 +42,7 @@ struct AdamArgs {
 	const uint32_t *d_count;
 	uint32_t max_count, clamp;
 	DeviceState *state;
+	int parity;
 	nrc_optimizer_entry *entries;
 	_Float16 *w16, *w16t, *use16;
 	float *w32, *use32;
@@ -71,7 +73,7 @@ hipError_t launch_fused_infer(int hidden, int encoding, const float *queries, co
                               uint32_t max_count, const _Float16 *w16, float *rgb, hipStream_t stream);
 hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *w16, float *out,
                                     hipStream_t stream);
-// 64-record chunks per workgroup for a step of n records (about one workgroup per two CUs)
+// 64-record chunks per workgroup for a step of n records (about one workgroup per CU)
 uint32_t fused_train_chunks(uint32_t n);
 hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t chunks, uint32_t *nparts, hipStream_t stream);
 hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,

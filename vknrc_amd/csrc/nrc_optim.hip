@@ -2,13 +2,15 @@
 // optimizer scalar recurrences (nrc_optimize.comp:32-54, nrc_train_prepare.comp:16-29).
 //
 // Each block owns 64 consecutive weights; its 256 threads sum the per-workgroup
-// partial gradients as float4 columns in 16 interleaved slices (four loads in
+// partial gradients as float4 columns in 16 interleaved slices (eight loads in
 // flight per thread) that are combined in a fixed order, so the reduced
 // gradient is bit-identical from run to run.  Weight counts are multiples of 64
 // and partial rows are 64-float aligned (the row stride holds the loss column and
-// its 63 padding floats), so the float4 loads stay in bounds.  The last
-// block to finish advances the scalar state (t, beta1^t, beta2^t, alpha^t,
-// alpha^{t-1}, loss scale, loss), which every block read at its start.
+// its 63 padding floats), so the float4 loads stay in bounds.  One extra block
+// reduces the loss column.  The scalar state (t, beta1^t, beta2^t, alpha^t,
+// alpha^{t-1}, loss scale, loss) is double-buffered by the step parity p: every
+// block reads pub[p]; block 0 writes pub[p ^ 1] except the loss fields, which
+// the loss block writes.
 #include "nrc_internal.hpp"
 
 namespace nrc {
@@ -16,12 +18,20 @@ namespace nrc {
 namespace {
 
 __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
-	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
-	if (count == 0) return; // nrc_optimize.comp:33-34: an empty batch changes nothing
 	const int t = threadIdx.x;
 	DeviceState *st = a.state;
-	const nrc_optimizer_state s0 = st->pub;
-	const bool skip = a.p.dynamic_scale && (st->overflow != 0);
+	const int p = a.parity;
+	const nrc_optimizer_state s0 = st->pub[p];
+	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
+	const bool loss_block = blockIdx.x == gridDim.x - 1;
+	if (count == 0) { // nrc_optimize.comp:33-34: an empty batch changes nothing
+		if (blockIdx.x == 0 && t == 0) {
+			st->pub[p ^ 1] = s0;
+			st->overflow[p ^ 1] = 0;
+		}
+		return;
+	}
+	const bool skip = a.p.dynamic_scale && (st->overflow[p] != 0);
 
 	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 4*col4 .. +3
 	const int col4 = t & 15, slice = t >> 4;
@@ -29,48 +39,54 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
 	__shared__ float4 red[16][16];
 	float4 G4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	if (!skip) {
-		const float4 *p = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
+		const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
 		const size_t ps4 = a.pstride / 4;
-		float4 acc[4];
+		float4 acc[8];
 #pragma unroll
-		for (int u = 0; u < 4; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		for (int u = 0; u < 8; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		uint32_t q = slice;
-		for (; q + 48 < a.nparts; q += 64) {
+		for (; q + 112 < a.nparts; q += 128) {
 #pragma unroll
-			for (int u = 0; u < 4; ++u) {
-				const float4 v = p[(q + 16 * u) * ps4];
+			for (int u = 0; u < 8; ++u) {
+				const float4 v = src[(q + 16 * u) * ps4];
 				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
 			}
 		}
-		for (; q < a.nparts; q += 16) {
-			const float4 v = p[q * ps4];
-			acc[0].x += v.x; acc[0].y += v.y; acc[0].z += v.z; acc[0].w += v.w;
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			if (q + 16 * u < a.nparts) {
+				const float4 v = src[(q + 16 * u) * ps4];
+				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
+			}
 		}
-		G4.x = (acc[0].x + acc[1].x) + (acc[2].x + acc[3].x);
-		G4.y = (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y);
-		G4.z = (acc[0].z + acc[1].z) + (acc[2].z + acc[3].z);
-		G4.w = (acc[0].w + acc[1].w) + (acc[2].w + acc[3].w);
+		G4 = acc[0];
+#pragma unroll
+		for (int u = 1; u < 8; ++u) {
+			G4.x += acc[u].x; G4.y += acc[u].y; G4.z += acc[u].z; G4.w += acc[u].w;
+		}
 	}
 	red[slice][col4] = G4;
 	__syncthreads();
-	const uint32_t i = w0 + (uint32_t)t;
-	if (blockIdx.x == gridDim.x - 1) {
-		// the extra block: column weight_count of the partials holds each workgroup's loss sum
-		if (t == 0 && !skip) {
-			const float *rf = reinterpret_cast<const float *>(&red[0][0]);
+	const float *rf = reinterpret_cast<const float *>(&red[0][0]);
+	if (loss_block) {
+		// column weight_count of the partials holds each workgroup's loss sum
+		if (t == 0) {
 			float L = 0.0f;
 #pragma unroll
 			for (int sl = 0; sl < 16; ++sl) L += rf[sl * 64];
-			st->loss_sum = L;
+			st->pub[p ^ 1].last_loss = skip ? s0.last_loss : L / (float)count;
+			st->pub[p ^ 1].last_count = skip ? s0.last_count : count;
 		}
-	} else if (t < 64 && !skip && i < a.weight_count) {
-		const float *rf = reinterpret_cast<const float *>(&red[0][0]);
+		return;
+	}
+	const uint32_t i = w0 + (uint32_t)t;
+	const float b1 = a.p.beta1, b2 = a.p.beta2, al = a.p.ema_alpha;
+	const float B1 = s0.beta1_t * b1, B2 = s0.beta2_t * b2; // nrc_train_prepare.comp:23-27
+	const float A1 = s0.alpha_t, A = s0.alpha_t * al;
+	if (t < 64 && !skip && i < a.weight_count) {
 		float G = 0.0f;
 #pragma unroll
 		for (int sl = 0; sl < 16; ++sl) G += rf[sl * 64 + t];
-		const float b1 = a.p.beta1, b2 = a.p.beta2, al = a.p.ema_alpha;
-		const float B1 = s0.beta1_t * b1, B2 = s0.beta2_t * b2; // nrc_train_prepare.comp:23-27
-		const float A1 = s0.alpha_t, A = s0.alpha_t * al;
 		float g = G / (float)count / s0.loss_scale; // nrc_optimize.comp:36
 		if (!isfinite(g)) g = 0.0f;                 // nrc_optimize.comp:37-38
 		nrc_optimizer_entry e = a.entries[i];
@@ -92,38 +108,37 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
 			a.use16[i] = (_Float16)u;
 		}
 	}
-	__syncthreads();
-	if (t == 0) {
-		__threadfence();
-		const uint32_t done = atomicAdd(&st->done_blocks, 1u);
-		if (done == gridDim.x - 1) {
-			__threadfence();
-			nrc_optimizer_state s = s0;
-			if (skip) {
-				s.loss_scale *= 0.5f;
-				s.clean_steps = 0;
-				s.skipped_steps += 1;
-			} else {
-				s.t += 1;
-				s.beta1_t *= a.p.beta1;
-				s.beta2_t *= a.p.beta2;
-				s.alpha_t_1 = s.alpha_t;
-				s.alpha_t *= a.p.ema_alpha;
-				s.last_loss = *(volatile float *)&st->loss_sum / (float)count;
-				s.last_count = count;
-				if (a.p.dynamic_scale) {
-					s.clean_steps += 1;
-					if (a.p.growth_interval > 0 && s.clean_steps >= a.p.growth_interval) {
-						s.loss_scale *= 2.0f;
-						s.clean_steps = 0;
-					}
+	if (blockIdx.x == 0 && t == 0) {
+		nrc_optimizer_state s = s0;
+		if (skip) {
+			s.loss_scale *= 0.5f;
+			s.clean_steps = 0;
+			s.skipped_steps += 1;
+		} else {
+			s.t += 1;
+			s.beta1_t = B1;
+			s.beta2_t = B2;
+			s.alpha_t_1 = A1;
+			s.alpha_t = A;
+			if (a.p.dynamic_scale) {
+				s.clean_steps += 1;
+				if (a.p.growth_interval > 0 && s.clean_steps >= a.p.growth_interval) {
+					s.loss_scale *= 2.0f;
+					s.clean_steps = 0;
 				}
 			}
-			st->pub = s;
-			st->overflow = 0;
-			st->done_blocks = 0;
-			__threadfence();
 		}
+		// every field but the loss pair, which the loss block writes
+		nrc_optimizer_state *d = &st->pub[p ^ 1];
+		d->t = s.t;
+		d->beta1_t = s.beta1_t;
+		d->beta2_t = s.beta2_t;
+		d->alpha_t = s.alpha_t;
+		d->alpha_t_1 = s.alpha_t_1;
+		d->loss_scale = s.loss_scale;
+		d->clean_steps = s.clean_steps;
+		d->skipped_steps = s.skipped_steps;
+		st->overflow[p ^ 1] = 0;
 	}
 }
 
@@ -156,9 +171,10 @@ __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 	s.skipped_steps = 0;
 	s.last_loss = 0.0f;
 	s.last_count = 0;
-	st->pub = s;
-	st->done_blocks = 0;
-	st->overflow = 0;
+	st->pub[0] = s;
+	st->pub[1] = s;
+	st->overflow[0] = 0;
+	st->overflow[1] = 0;
 }
 
 } // namespace

@@ -49,10 +49,25 @@ template <int H> __device__ __forceinline__ void forward_f32(const float *sW, fl
 	}
 }
 
-template <int H> __device__ __forceinline__ void stage_weights(float *sW, const float *__restrict__ w32) {
-	constexpr int WC = weight_count(H);
-	for (int i = threadIdx.x * 4; i < WC; i += blockDim.x * 4)
-		*reinterpret_cast<float4 *>(sW + i) = *reinterpret_cast<const float4 *>(w32 + i);
+// All weights into LDS by a BLOCK-thread block; loads go out four at a time
+// before their LDS stores (one memory round trip per four float4 per thread).
+template <int H, int BLOCK> __device__ __forceinline__ void stage_weights(float *sW, const float *__restrict__ w32) {
+	constexpr int kVec = weight_count(H) / 4;
+	const float4 *src = reinterpret_cast<const float4 *>(w32);
+	float4 *dst = reinterpret_cast<float4 *>(sW);
+#pragma unroll
+	for (int b = 0; b < kVec; b += 4 * BLOCK) {
+		const int j0 = b + threadIdx.x, j1 = j0 + BLOCK, j2 = j0 + 2 * BLOCK, j3 = j0 + 3 * BLOCK;
+		float4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
+		if (j0 < kVec) v0 = src[j0];
+		if (j1 < kVec) v1 = src[j1];
+		if (j2 < kVec) v2 = src[j2];
+		if (j3 < kVec) v3 = src[j3];
+		if (j0 < kVec) dst[j0] = v0;
+		if (j1 < kVec) dst[j1] = v1;
+		if (j2 < kVec) dst[j2] = v2;
+		if (j3 < kVec) dst[j3] = v3;
+	}
 	__syncthreads();
 }
 
@@ -72,7 +87,7 @@ __global__ __launch_bounds__(512) void plain_infer_kernel(const float *__restric
                                                           const uint32_t *__restrict__ d_count, uint32_t max_count,
                                                           const float *__restrict__ w32, float *__restrict__ rgb) {
 	__shared__ __attribute__((aligned(16))) float sW[weight_count(H)];
-	stage_weights<H>(sW, w32);
+	stage_weights<H, 512>(sW, w32);
 	const uint32_t count = effective_count(d_count, max_count, max_count);
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= count) return;
@@ -93,16 +108,16 @@ template <int H, int ENC, int LOSS>
 __global__ __launch_bounds__(256) void plain_fwdbwd_kernel(const float *__restrict__ records,
                                                            const uint32_t *__restrict__ d_count, uint32_t max_count,
                                                            uint32_t clamp, const float *__restrict__ w32,
-                                                           const DeviceState *__restrict__ st, float *__restrict__ acts,
-                                                           float *__restrict__ deltas, float *__restrict__ sample_loss,
-                                                           uint32_t ldn) {
+                                                           const DeviceState *__restrict__ st, int parity,
+                                                           float *__restrict__ acts, float *__restrict__ deltas,
+                                                           float *__restrict__ sample_loss, uint32_t ldn) {
 	__shared__ __attribute__((aligned(16))) float sW[weight_count(H)];
-	stage_weights<H>(sW, w32);
+	stage_weights<H, 256>(sW, w32);
 	const uint32_t count = effective_count(d_count, max_count, clamp);
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= count) return;
 	const size_t slab = (size_t)kWidth * ldn;
-	const float s = st->pub.loss_scale;
+	const float s = st->pub[parity].loss_scale;
 	const float *r = records + (size_t)n * kTrainFloats;
 	float rec[kQueryFloats];
 #pragma unroll
@@ -204,7 +219,7 @@ __global__ __launch_bounds__(256) void plain_dw_kernel(const uint32_t *__restric
                                                        const float *__restrict__ deltas,
                                                        const float *__restrict__ sample_loss, uint32_t ldn,
                                                        float *__restrict__ partials, uint32_t pstride,
-                                                       DeviceState *__restrict__ st) {
+                                                       uint32_t *__restrict__ overflow) {
 	constexpr int WC = weight_count(H);
 	__shared__ float sa[kWidth][kWidth + 1];
 	__shared__ float sd[4][kWidth];
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(256) void plain_dw_kernel(const uint32_t *__restric
 	float *pp = partials + (size_t)blockIdx.y * pstride;
 	if (w < (uint32_t)WC) {
 		pp[w] = acc;
-		if (!isfinite(acc)) atomicOr(&st->overflow, 1u);
+		if (!isfinite(acc)) atomicOr(overflow, 1u);
 	}
 	if (blockIdx.x == 0) {
 		float v = 0.0f;
@@ -297,7 +312,7 @@ hipError_t plain_infer_h(int enc, const float *q, const uint32_t *dc, uint32_t m
 template <int H, int ENC, int LOSS> void plain_fwdbwd_launch(const TrainArgs &a, hipStream_t s) {
 	const dim3 grid((a.max_count + 255) / 256), block(256);
 	plain_fwdbwd_kernel<H, ENC, LOSS><<<grid, block, 0, s>>>(a.records, a.d_count, a.max_count, a.clamp, a.w32,
-	                                                          a.state, a.acts, a.deltas, a.sample_loss, a.ldn);
+	                                                          a.state, a.parity, a.acts, a.deltas, a.sample_loss, a.ldn);
 }
 
 template <int H, int ENC> void plain_fwdbwd_loss(const TrainArgs &a, hipStream_t s) {
@@ -313,7 +328,7 @@ template <int H> void plain_gradient_h(const TrainArgs &a, uint32_t nparts, hipS
 	}
 	const dim3 grid((weight_count(H) + 255) / 256, nparts), block(256);
 	plain_dw_kernel<H><<<grid, block, 0, s>>>(a.d_count, a.max_count, a.clamp, a.acts, a.deltas, a.sample_loss, a.ldn,
-	                                          a.partials, a.pstride, a.state);
+	                                          a.partials, a.pstride, &a.state->overflow[a.parity]);
 }
 
 } // namespace

@@ -330,47 +330,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 // kernel reduces the partials in a fixed order.
 constexpr int kGradWaves = 8;
 constexpr int kGradRecords = 16 * kGradWaves; // records per chunk
-constexpr int kGLdw = 68;                     // halves per LDS weight row (64 + 4 padding)
 constexpr int kGLdn = kGradRecords + 8;       // halves per LDS activation row
 
+// Weights sit in LDS as unpadded 128-byte rows of 8 16-byte granules, granule j
+// of row R stored at slot j ^ (R & 7) of that row.  The XOR spreads the 16 rows
+// an A-fragment read touches over all banks, and keeps every granule a whole
+// 16-byte slot, so the rows can be filled by direct global->LDS loads.
+__device__ __forceinline__ int swz(int row, int col) {
+	return row * 128 + ((((col >> 3) ^ (row & 7))) << 4) + ((col & 7) << 1);
+}
+
 template <int H> struct GradSmem {
-	_Float16 w[(H * kWidth + kOut) * kGLdw];
-	_Float16 wt[H * kWidth * kGLdw];
+	_Float16 w[(H * kWidth + kOut) * kWidth];
+	_Float16 wt[H * kWidth * kWidth];
 	_Float16 act[2][kWidth * kGLdn];   // a_l feature-major [feature][record], double-buffered
 	_Float16 delta[2][kWidth * kGLdn]; // delta_l feature-major [feature][record], double-buffered
 	float tgt[kGradRecords * 3];
 	float loss[kGradRecords];
 };
 
-// Row-major [ROWS][64] fp16 weights -> LDS rows of kGLdw halves, by a 512-thread
-// block: loads go out in batches before their LDS stores (8-byte stores, since
-// the 136-byte rows are 8-byte aligned).
-template <int ROWS> __device__ __forceinline__ void stage_rows_grad(_Float16 *sW, const _Float16 *__restrict__ w16) {
-	constexpr int kChunks = ROWS * 8;
-	const uint4 *src = reinterpret_cast<const uint4 *>(w16);
-	auto put = [&](int ci, uint4 v) __attribute__((always_inline)) {
-		uint2 *d = reinterpret_cast<uint2 *>(sW + (ci >> 3) * kGLdw + (ci & 7) * 8);
-		d[0] = make_uint2(v.x, v.y);
-		d[1] = make_uint2(v.z, v.w);
-	};
-#pragma unroll
-	for (int b = 0; b < kChunks; b += 2048) {
-		const int c0 = b + threadIdx.x, c1 = c0 + 512, c2 = c0 + 1024, c3 = c0 + 1536;
-		uint4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
-		if (c0 < kChunks) v0 = src[c0];
-		if (c1 < kChunks) v1 = src[c1];
-		if (c2 < kChunks) v2 = src[c2];
-		if (c3 < kChunks) v3 = src[c3];
-		if (c0 < kChunks) put(c0, v0);
-		if (c1 < kChunks) put(c1, v1);
-		if (c2 < kChunks) put(c2, v2);
-		if (c3 < kChunks) put(c3, v3);
+// Row-major [ROWS][64] fp16 weights -> the swizzled LDS rows, with direct
+// global->LDS 16-byte loads (global_load_lds_dwordx4 puts lane i of a wave at
+// the wave-uniform LDS base + 16 i).  Wave-instruction k fills slots
+// 64k .. 64k+63; slot p holds granule (p & 7) ^ (row & 7) of row p >> 3.  All
+// loads of the block are in flight together; the caller waits (vmcnt) and
+// synchronises before reading.
+template <int ROWS> __device__ __forceinline__ void stage_rows_lds(_Float16 *lds, const _Float16 *__restrict__ w16) {
+	constexpr int kGran = ROWS * 8;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	for (int k = wave; k * 64 < kGran; k += kGradWaves) {
+		const int p = k * 64 + lane;
+		if (p < kGran) {
+			const int row = p >> 3, j = (p & 7) ^ (row & 7);
+			__builtin_amdgcn_global_load_lds(
+			    reinterpret_cast<const void *>(w16 + (row * 8 + j) * 8),
+			    (__attribute__((address_space(3))) void *)(reinterpret_cast<char *>(lds) + k * 64 * 16), 16, 0, 0);
+		}
 	}
 }
 
 __device__ __forceinline__ half8_t load_afrag_g(const _Float16 *sW, int row, int c, int g) {
-	const half4_t lo = *reinterpret_cast<const half4_t *>(sW + row * kGLdw + 32 * c + 4 * g);
-	const half4_t hi = *reinterpret_cast<const half4_t *>(sW + row * kGLdw + 32 * c + 16 + 4 * g);
+	const char *b = reinterpret_cast<const char *>(sW);
+	const half4_t lo = *reinterpret_cast<const half4_t *>(b + swz(row, 32 * c + 4 * g));
+	const half4_t hi = *reinterpret_cast<const half4_t *>(b + swz(row, 32 * c + 16 + 4 * g));
 	half8_t a;
 	a[0] = lo[0]; a[1] = lo[1]; a[2] = lo[2]; a[3] = lo[3];
 	a[4] = hi[0]; a[5] = hi[1]; a[6] = hi[2]; a[7] = hi[3];
@@ -428,8 +430,9 @@ __global__ __launch_bounds__(512) void fused_grad_kernel(const float *__restrict
 	};
 	load_rec(0); // in flight while the weights are staged
 
-	stage_rows_grad<H * kWidth + kOut>(S.w, w16);
-	stage_rows_grad<H * kWidth>(S.wt, w16t);
+	stage_rows_lds<H * kWidth + kOut>(S.w, w16);
+	stage_rows_lds<H * kWidth>(S.wt, w16t);
+	__builtin_amdgcn_s_waitcnt(0); // this wave's direct-to-LDS loads (and the record) have landed
 
 	float4_t accW[MULTI ? H : 1][2]; // multi-chunk accumulators of this wave's two dW tiles per layer
 	float4_t accO = (float4_t)0.0f;   // dW_H^T tile (waves 0..3)
@@ -577,9 +580,10 @@ __global__ __launch_bounds__(512) void fused_grad_kernel(const float *__restrict
 				half8_t a = (half8_t)0;
 				if (g == 0) {
 					const int m = 16 * mt + r;
-					a[0] = S.w[(H * kWidth + 0) * kGLdw + m];
-					a[1] = S.w[(H * kWidth + 1) * kGLdw + m];
-					a[2] = S.w[(H * kWidth + 2) * kGLdw + m];
+					const char *wb = reinterpret_cast<const char *>(S.w);
+					a[0] = *reinterpret_cast<const _Float16 *>(wb + swz(H * kWidth + 0, m));
+					a[1] = *reinterpret_cast<const _Float16 *>(wb + swz(H * kWidth + 1, m));
+					a[2] = *reinterpret_cast<const _Float16 *>(wb + swz(H * kWidth + 2, m));
 				}
 				acc[mt] = mfma(a, Bg, (float4_t)0.0f);
 			}

@@ -79,6 +79,9 @@ def main() -> int:
     ap.add_argument("--plain-reps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after timing, run one elementwise copy (torch.neg) of 256 MiB fp32 -> 256 MiB, the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the NRC hot path is measured per GPU "
@@ -179,6 +182,12 @@ def main() -> int:
                      "traffic": None,
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
     }
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(q, recs, args.cpu_seconds)

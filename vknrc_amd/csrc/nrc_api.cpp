@@ -19,6 +19,7 @@ struct nrc_cache {
 	uint32_t wc = 0;
 	nrc_optimizer_entry *entries = nullptr;
 	_Float16 *w16 = nullptr, *w16t = nullptr, *use16 = nullptr;
+	_Float16 *wf = nullptr, *wtf = nullptr; // fragment-major copies for the fused gradient
 	float *w32 = nullptr, *use32 = nullptr;
 	DeviceState *state = nullptr;
 	float *partials = nullptr;
@@ -99,7 +100,9 @@ void host_init_weights(const nrc_cache_desc *d, uint64_t seed, float *w) {
 
 // Uploads entries + the derived fp16/fp32 weight copies (ResetMLPBuffers, VkNRCState.cpp:46-88).
 nrc_status upload_entries(nrc_cache *c, const nrc_optimizer_entry *entries) {
+	const int H = (int)c->desc.hidden_layers;
 	std::vector<_Float16> w16(c->wc), w16t(c->wc), u16(c->wc);
+	std::vector<_Float16> wf(nrc::fwd_frag_halves(H), (_Float16)0.0f), wtf(nrc::bwd_frag_halves(H), (_Float16)0.0f);
 	std::vector<float> w32(c->wc), u32(c->wc);
 	const uint32_t hid = c->desc.hidden_layers * 4096u;
 	for (uint32_t i = 0; i < c->wc; ++i) {
@@ -109,11 +112,14 @@ nrc_status upload_entries(nrc_cache *c, const nrc_optimizer_entry *entries) {
 		u16[i] = (_Float16)u32[i];
 		// transposed hidden layers [l][in][out] for the fused backward
 		w16t[i < hid ? ((i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)) : i] = w16[i];
+		nrc::frag_scatter(i, w16[i], wf.data(), wtf.data());
 	}
 	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
 	NRC_HIP(hipMemcpy(c->entries, entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyHostToDevice), "upload entries");
 	NRC_HIP(hipMemcpy(c->w16, w16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->w16t, w16t.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->wf, wf.data(), 2 * wf.size(), hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->wtf, wtf.data(), 2 * wtf.size(), hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->use16, u16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload use weights");
 	NRC_HIP(hipMemcpy(c->w32, w32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->use32, u32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload use weights");
@@ -137,7 +143,7 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 void free_cache(nrc_cache *c) {
 	if (!c) return;
 	(void)hipSetDevice(c->device);
-	void *ptrs[] = {c->entries, c->w16, c->w16t, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
+	void *ptrs[] = {c->entries, c->w16, c->w16t, c->wf, c->wtf, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	delete c;
@@ -226,11 +232,14 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	c->desc = *desc;
 	c->wc = nrc_weight_count(desc->hidden_layers);
 	c->pstride = ((c->wc + 1) + 63u) & ~63u;
-	c->part_cap = (desc->train_batch_size + nrc::kFusedTrainSamples - 1) / nrc::kFusedTrainSamples;
+	// one partial row per 64 records covers every fused-gradient plan (and the plain path's 512)
+	c->part_cap = (desc->train_batch_size + nrc::kFusedSmallRecords - 1) / nrc::kFusedSmallRecords;
 	hipError_t e = hipGetDevice(&c->device);
 	if (e == hipSuccess) e = hipMalloc(&c->entries, sizeof(nrc_optimizer_entry) * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w16, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w16t, 2 * c->wc);
+	if (e == hipSuccess) e = hipMalloc(&c->wf, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
+	if (e == hipSuccess) e = hipMalloc(&c->wtf, 2 * (size_t)nrc::bwd_frag_halves((int)desc->hidden_layers));
 	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
@@ -305,15 +314,16 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	a.w32 = c->w32;
 	a.w16 = c->w16;
 	a.w16t = c->w16t;
+	a.wf = c->wf;
+	a.wtf = c->wtf;
 	a.state = c->state;
 	a.parity = c->parity;
 	a.partials = c->partials;
 	a.pstride = c->pstride;
 	// partial-gradient slots the launch will write, checked before anything runs
 	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
-	const uint32_t chunks = path == NRC_PATH_PLAIN_F32 ? 1u : nrc::fused_train_chunks(n);
-	const uint32_t per = path == NRC_PATH_PLAIN_F32 ? nrc::kPlainChunk : nrc::kFusedTrainSamples * chunks;
-	const uint32_t need = (n + per - 1) / per;
+	const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+	const uint32_t need = path == NRC_PATH_PLAIN_F32 ? (n + nrc::kPlainChunk - 1) / nrc::kPlainChunk : plan.blocks;
 	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
 	uint32_t nparts = 0;
 	hipError_t e;
@@ -326,7 +336,8 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 		a.ldn = c->ldn;
 		e = nrc::launch_plain_gradient(a, &nparts, s);
 	} else {
-		e = nrc::launch_fused_gradient(a, chunks, &nparts, s);
+		e = nrc::launch_fused_gradient(a, plan, s);
+		nparts = plan.blocks;
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
 	nrc::AdamArgs o{};
@@ -341,6 +352,8 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	o.entries = c->entries;
 	o.w16 = c->w16;
 	o.w16t = c->w16t;
+	o.wf = c->wf;
+	o.wtf = c->wtf;
 	o.use16 = c->use16;
 	o.w32 = c->w32;
 	o.use32 = c->use32;
@@ -439,20 +452,29 @@ nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint
 	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 	const uint32_t wc = nrc_weight_count(hidden);
 	const uint32_t pstride = ((wc + 1) + 63u) & ~63u;
-	const uint32_t nb = n == 0 ? 1 : (n + nrc::kFusedTrainSamples - 1) / nrc::kFusedTrainSamples;
+	// enough partial rows for any plan: one per 64 records
+	const uint32_t nb = n == 0 ? 1 : (n + nrc::kFusedSmallRecords - 1) / nrc::kFusedSmallRecords;
+	const size_t nf = nrc::fwd_frag_halves((int)hidden), nt = nrc::bwd_frag_halves((int)hidden);
 	float *partials = nullptr;
-	_Float16 *wt = nullptr;
+	_Float16 *wt = nullptr, *wf = nullptr, *wtf = nullptr;
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s), "gradient scratch");
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wt), 2 * (size_t)wc, s), "gradient scratch");
+	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wf), 2 * nf, s), "gradient scratch");
+	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wtf), 2 * nt, s), "gradient scratch");
 	uint32_t nparts = 0;
 	const _Float16 *w = reinterpret_cast<const _Float16 *>(d_w);
 	hipError_t e = nrc::launch_transpose_hidden((int)hidden, w, wt, s);
+	if (e == hipSuccess) e = hipMemsetAsync(wf, 0, 2 * nf, s);
+	if (e == hipSuccess) e = hipMemsetAsync(wtf, 0, 2 * nt, s);
+	if (e == hipSuccess) e = nrc::launch_make_frags((int)hidden, w, wf, wtf, s);
 	if (e == hipSuccess)
 		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, w, wt,
-		                                   partials, pstride, &nparts, s);
+		                                   wf, wtf, partials, pstride, &nparts, s);
 	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
 	hipError_t e2 = hipFreeAsync(partials, s);
 	if (e2 == hipSuccess) e2 = hipFreeAsync(wt, s);
+	if (e2 == hipSuccess) e2 = hipFreeAsync(wf, s);
+	if (e2 == hipSuccess) e2 = hipFreeAsync(wtf, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16");
 	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_gradient_f16 free");
 	return NRC_OK;

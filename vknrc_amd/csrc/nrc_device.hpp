@@ -15,6 +15,31 @@ constexpr int kTrainFloats = 17;
 
 __host__ __device__ constexpr int weight_count(int hidden) { return hidden * kWidth * kWidth + kOut * kWidth; }
 
+// Fragment-major fp16 weight copies, the operand layout of the gradient kernel's
+// v_mfma_f32_16x16x32_f16 A fragments: block b = 8 * layer + 2 * mt + c holds
+// 64 lanes x 8 halves, lane 16g + r carrying row 16 mt + r at the columns
+//   32c + 4g + j (j < 4),  32c + 16 + 4g + j - 4 (j >= 4)
+// so each lane loads one fragment with a single 16-byte load.
+//   forward copy : W_l for l < H (blocks 0 .. 8H-1), then the 3x64 output layer
+//                  in blocks 8H, 8H+1 (rows 3..15 stay zero)
+//   backward copy: W_l^T for l = 1 .. H-1 in blocks 8(l-1) ..; W_H^T (64 x 3)
+//                  in the c = 0 blocks of layer slot H-1 (K columns 3..31 stay zero)
+__host__ __device__ constexpr uint32_t fwd_frag_halves(int hidden) { return (uint32_t)(hidden * 8 + 2) * 512u; }
+__host__ __device__ constexpr uint32_t bwd_frag_halves(int hidden) { return (uint32_t)hidden * 4096u; }
+
+// Position of element (row, col) of the 64-column matrix in layer slot l.
+__host__ __device__ inline uint32_t frag_pos(uint32_t l, uint32_t row, uint32_t col) {
+	const uint32_t c = col >> 5, w = col & 31u, g = (w & 15u) >> 2, j = (w & 3u) + ((w >> 4) << 2);
+	return ((((l * 4u + (row >> 4)) * 2u + c) * 64u) + g * 16u + (row & 15u)) * 8u + j;
+}
+
+// Scatters weight i (row-major layer order) into both fragment copies.
+template <typename T> __host__ __device__ inline void frag_scatter(uint32_t i, T h, T *wf, T *wtf) {
+	const uint32_t l = i >> 12, row = (i >> 6) & 63u, col = i & 63u;
+	wf[frag_pos(l, row, col)] = h;
+	if (l >= 1) wtf[frag_pos(l - 1, col, row)] = h;
+}
+
 // Device-resident optimizer state, double-buffered by a parity the host flips
 // on every launched train step: step p reads pub[p] and writes pub[p ^ 1], and
 // its gradient kernels flag non-finite partials in overflow[p] (the optimizer

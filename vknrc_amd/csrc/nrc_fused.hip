@@ -673,6 +673,271 @@ __global__ __launch_bounds__(512) void fused_grad_kernel(const float *__restrict
 	if (bad && overflow) atomicOr(overflow, 1u);
 }
 
+// ================================================ gradient, register-resident weights
+//
+// One workgroup = 4 waves (one per SIMD) and one 64-record chunk; wave w runs
+// records 16w .. 16w+15 through the whole network with every A fragment it needs
+// held in registers, loaded once with 16-byte loads from the fragment-major
+// copies: W_l for the forward, W_l^T for the backward (312 registers for H = 5
+// of the 512 a wave owns at one wave per SIMD).  Waves write their activations
+// a_l and deltas feature-major into LDS as they produce them, so the forward and
+// the backward chain run without any barrier between waves.  After the one
+// barrier, wave w computes its 2x2 block of dW_l^T tiles (input tiles 2(w&1)..,
+// output tiles 2(w>>1)..) for every layer from LDS, K = the 64 records, and
+// stores them (float4 per lane) to its workgroup's partial-gradient row.
+constexpr int kSmallLd = (int)kFusedSmallRecords + 8; // halves per LDS row
+
+template <int H> struct SmallGradSmem {
+	_Float16 act[H + 1][kWidth * kSmallLd]; // a_l, [feature][record]
+	_Float16 dlt[H][kWidth * kSmallLd];     // delta_{l+1} = dL/dz at layer l's output, [unit][record]
+	_Float16 go[16 * kSmallLd];             // output gradient rows 0..2; rows 3..15 stay zero
+	float lsum[4];
+};
+
+// writes this lane's 16 values of a B operand (record column col) feature-major, row stride LD
+template <int LD> __device__ __forceinline__ void store_bcol_ld(_Float16 *dst, const half8_t *b, int g, int col) {
+#pragma unroll
+	for (int c = 0; c < 2; ++c)
+#pragma unroll
+		for (int j = 0; j < 8; ++j) dst[bfeat(c, g, j) * LD + col] = b[c][j];
+}
+
+// delta = relu_bwd(acc) against the activations in B-operand layout, packed to fp16
+__device__ __forceinline__ void relu_bwd_pack(const float4_t *acc, const half8_t *act, half8_t *d) {
+#pragma unroll
+	for (int c = 0; c < 2; ++c)
+#pragma unroll
+		for (int i = 0; i < 4; ++i) {
+			d[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)act[c][i]);
+			d[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)act[c][4 + i]);
+		}
+}
+
+template <int H, int ENC, int LOSS, bool F16IN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void fused_grad_small_kernel(
+    const float *__restrict__ records, const _Float16 *__restrict__ fin, const float *__restrict__ ftgt,
+    const uint32_t *__restrict__ d_count, uint32_t max_count, uint32_t clamp, const _Float16 *__restrict__ wf,
+    const _Float16 *__restrict__ wtf, const DeviceState *__restrict__ st, int parity, float *__restrict__ partials,
+    uint32_t pstride, uint32_t *__restrict__ overflow) {
+	constexpr int WC = weight_count(H);
+	constexpr int R = (int)kFusedSmallRecords;
+	constexpr int LD = kSmallLd;
+	extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+	SmallGradSmem<H> &S = *reinterpret_cast<SmallGradSmem<H> *>(smem_raw);
+
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	const int g = lane >> 4, r = lane & 15;
+	const int col = 16 * wave + r; // record column this lane runs through the network
+	const uint32_t count = F16IN ? max_count : effective_count(d_count, max_count, clamp);
+	const uint32_t base = blockIdx.x * (uint32_t)R;
+	float *part = partials + (size_t)blockIdx.x * pstride;
+	const int it0 = 2 * (wave & 1), ot0 = 2 * (wave >> 1); // this wave's dW tiles
+
+	auto store_tile = [&](int l, int it, int ot, float4_t v) __attribute__((always_inline)) {
+		*reinterpret_cast<float4_t *>(part + l * kWidth * kWidth + (16 * ot + r) * kWidth + 16 * it + 4 * g) = v;
+	};
+	if (base >= count) { // uniform: no valid record in this chunk, so its partial row is zero
+#pragma unroll
+		for (int l = 0; l < H; ++l)
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int o = 0; o < 2; ++o) store_tile(l, it0 + i, ot0 + o, (float4_t)0.0f);
+		if (r < kOut)
+			*reinterpret_cast<float4_t *>(part + H * kWidth * kWidth + r * kWidth + 16 * wave + 4 * g) = (float4_t)0.0f;
+		if (tid == 0) part[WC] = 0.0f;
+		return;
+	}
+	const float scale = (F16IN || st == nullptr) ? 1.0f : st->pub[parity].loss_scale;
+	const uint32_t last = count - 1;
+
+	// ---- loads, in the order they are needed: the record this lane encodes
+	// (base + lane), the targets of the record it runs (base + col), then every
+	// weight fragment.  Indices are clamped so the loads are unconditional.
+	const uint32_t n_enc = base + lane < last ? base + lane : last;
+	const uint32_t n_run = base + col < last ? base + col : last;
+	float q[kQueryFloats];
+	uint4 fq[2];
+	float tg[3];
+	if constexpr (F16IN) {
+		const uint4 *src = reinterpret_cast<const uint4 *>(fin + (size_t)n_enc * kWidth + 16 * wave);
+		fq[0] = src[0];
+		fq[1] = src[1];
+#pragma unroll
+		for (int c = 0; c < 3; ++c) tg[c] = ftgt[(size_t)n_run * 3 + c];
+	} else {
+		const float *rp = records + (size_t)n_enc * kTrainFloats;
+#pragma unroll
+		for (int i = 0; i < kQueryFloats; ++i) q[i] = rp[i];
+		const float *tp = records + (size_t)n_run * kTrainFloats + kQueryFloats;
+#pragma unroll
+		for (int c = 0; c < 3; ++c) tg[c] = tp[c];
+	}
+	const half8_t *wfv = reinterpret_cast<const half8_t *>(wf);
+	const half8_t *wtv = reinterpret_cast<const half8_t *>(wtf);
+	half8_t Af[H][4][2], Ao[2], At[H - 1][4][2], Aot[4];
+#pragma unroll
+	for (int l = 0; l < H; ++l)
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+			for (int c = 0; c < 2; ++c) Af[l][mt][c] = wfv[((l * 4 + mt) * 2 + c) * 64 + lane];
+#pragma unroll
+	for (int c = 0; c < 2; ++c) Ao[c] = wfv[(H * 8 + c) * 64 + lane];
+#pragma unroll
+	for (int mt = 0; mt < 4; ++mt) Aot[mt] = wtv[((H - 1) * 4 + mt) * 2 * 64 + lane];
+#pragma unroll
+	for (int l = 0; l < H - 1; ++l)
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+			for (int c = 0; c < 2; ++c) At[l][mt][c] = wtv[((l * 4 + mt) * 2 + c) * 64 + lane];
+
+	// rows 3..15 of the output-gradient operand stay zero
+	for (int i = tid; i < 13 * 32; i += 256)
+		reinterpret_cast<uint32_t *>(S.go + (3 + i / 32) * LD)[i % 32] = 0u;
+
+	// ---- encode: wave w computes feature block w (features 16w ..) of record base + lane
+	{
+		_Float16 *a0 = S.act[0] + lane;
+		const bool valid = base + lane < count;
+		if constexpr (F16IN) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const half8_t v = __builtin_bit_cast(half8_t, fq[h]);
+#pragma unroll
+				for (int j = 0; j < 8; ++j) a0[(16 * wave + 8 * h + j) * LD] = valid ? v[j] : (_Float16)0.0f;
+			}
+		} else {
+			auto put = [&](auto qc) __attribute__((always_inline)) {
+				constexpr int qb = decltype(qc)::value;
+				float f[16];
+				encode_block<ENC>(q, qb, f);
+#pragma unroll
+				for (int j = 0; j < 16; ++j) a0[(16 * qb + j) * LD] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+			};
+			switch (wave) {
+			case 0: put(std::integral_constant<int, 0>{}); break;
+			case 1: put(std::integral_constant<int, 1>{}); break;
+			case 2: put(std::integral_constant<int, 2>{}); break;
+			default: put(std::integral_constant<int, 3>{}); break;
+			}
+		}
+	}
+	__syncthreads();
+
+	// ---- forward for this wave's 16 records; a_l kept as B operands and written feature-major
+	half8_t Bs[H + 1][2];
+#pragma unroll
+	for (int c = 0; c < 2; ++c)
+#pragma unroll
+		for (int j = 0; j < 8; ++j) Bs[0][c][j] = S.act[0][bfeat(c, g, j) * LD + col];
+#pragma unroll
+	for (int l = 0; l < H; ++l) {
+		float4_t acc[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
+#pragma unroll
+		for (int c = 0; c < 2; ++c)
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(Af[l][mt][c], Bs[l][c], acc[mt]);
+#pragma unroll
+		for (int c = 0; c < 2; ++c) Bs[l + 1][c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
+		store_bcol_ld<LD>(S.act[l + 1], Bs[l + 1], g, col);
+	}
+	float4_t o = (float4_t)0.0f;
+#pragma unroll
+	for (int c = 0; c < 2; ++c) o = mfma(Ao[c], Bs[H][c], o);
+
+	// ---- loss gradient: lanes g == 0 hold the 3 outputs of record col
+	half8_t Bg = (half8_t)0;
+	float lv = 0.0f;
+	if (g == 0) {
+		const bool valid = base + col < count;
+		const float oo[3] = {o[0], o[1], o[2]};
+		float go[3];
+		const float v = loss_grad<LOSS>(oo, tg, scale, go);
+		lv = valid ? v : 0.0f;
+#pragma unroll
+		for (int c = 0; c < kOut; ++c) {
+			Bg[c] = valid ? (_Float16)go[c] : (_Float16)0.0f;
+			S.go[c * LD + col] = Bg[c];
+		}
+	}
+	// ---- backward chain: delta_H = (W_H^T g_o) . 1[a_H > 0], then delta_l for l = H-1 .. 1
+	half8_t Bd[2];
+	{
+		float4_t acc[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(Aot[mt], Bg, (float4_t)0.0f);
+		relu_bwd_pack(acc, Bs[H], Bd);
+		store_bcol_ld<LD>(S.dlt[H - 1], Bd, g, col);
+	}
+#pragma unroll
+	for (int l = H - 1; l >= 1; --l) {
+		float4_t acc[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
+#pragma unroll
+		for (int c = 0; c < 2; ++c)
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(At[l - 1][mt][c], Bd[c], acc[mt]);
+		relu_bwd_pack(acc, Bs[l], Bd);
+		store_bcol_ld<LD>(S.dlt[l - 1], Bd, g, col);
+	}
+	// this wave's loss sum, in a fixed order
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1) lv += __shfl_xor(lv, off);
+	if (lane == 0) S.lsum[wave] = lv;
+	__syncthreads();
+
+	// ---- dW_l^T tiles of this wave over the chunk's records, straight to the partial row
+	bool bad = false;
+#pragma unroll
+	for (int l = 0; l < H; ++l) {
+		float4_t t[2][2];
+#pragma unroll
+		for (int i = 0; i < 2; ++i) t[i][0] = t[i][1] = (float4_t)0.0f;
+#pragma unroll
+		for (int kc = 0; kc < R / 32; ++kc) {
+			half8_t a[2], b[2];
+#pragma unroll
+			for (int i = 0; i < 2; ++i) {
+				a[i] = *reinterpret_cast<const half8_t *>(S.act[l] + (16 * (it0 + i) + r) * LD + 32 * kc + 8 * g);
+				b[i] = *reinterpret_cast<const half8_t *>(S.dlt[l] + (16 * (ot0 + i) + r) * LD + 32 * kc + 8 * g);
+			}
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int o2 = 0; o2 < 2; ++o2) t[i][o2] = mfma(a[i], b[o2], t[i][o2]);
+		}
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+#pragma unroll
+			for (int o2 = 0; o2 < 2; ++o2) {
+				store_tile(l, it0 + i, ot0 + o2, t[i][o2]);
+#pragma unroll
+				for (int e = 0; e < 4; ++e) bad |= !isfinite(t[i][o2][e]);
+			}
+	}
+	{ // dW_H^T: input tile `wave` against the output-gradient rows
+		float4_t t = (float4_t)0.0f;
+#pragma unroll
+		for (int kc = 0; kc < R / 32; ++kc) {
+			const half8_t a = *reinterpret_cast<const half8_t *>(S.act[H] + (16 * wave + r) * LD + 32 * kc + 8 * g);
+			const half8_t b = *reinterpret_cast<const half8_t *>(S.go + r * LD + 32 * kc + 8 * g);
+			t = mfma(a, b, t);
+		}
+		if (r < kOut) {
+			*reinterpret_cast<float4_t *>(part + H * kWidth * kWidth + r * kWidth + 16 * wave + 4 * g) = t;
+#pragma unroll
+			for (int e = 0; e < 4; ++e) bad |= !isfinite(t[e]);
+		}
+	}
+	if (tid == 0) part[WC] = (S.lsum[0] + S.lsum[1]) + (S.lsum[2] + S.lsum[3]);
+	if (overflow && __any(bad) && lane == 0) atomicOr(overflow, 1u);
+}
+
 // N-tiles each wave runs through the layers together (1 or 2; both measure the
 // same, and 1 needs no register spill); VKNRC_INFER_TILES overrides.
 int infer_tiles() {
@@ -718,23 +983,47 @@ hipError_t grad_launch(const float *rec, const _Float16 *fin, const float *ftgt,
 	return hipGetLastError();
 }
 
-template <int H, int ENC>
-hipError_t grad_enc(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
-	if (a.loss == NRC_LOSS_L2)
-		return grad_launch<H, ENC, NRC_LOSS_L2, false>(a.records, nullptr, nullptr, a.d_count, a.max_count, a.clamp,
-		                                                chunks, a.w16, a.w16t, a.state, a.parity, a.partials,
-		                                                a.pstride, &a.state->overflow[a.parity], nb, s);
-	return grad_launch<H, ENC, NRC_LOSS_RELATIVE_L2_LUMINANCE, false>(a.records, nullptr, nullptr, a.d_count,
-	                                                                  a.max_count, a.clamp, chunks, a.w16, a.w16t,
-	                                                                  a.state, a.parity, a.partials, a.pstride,
-	                                                                  &a.state->overflow[a.parity], nb, s);
+template <int H, int ENC, int LOSS, bool F16IN>
+hipError_t grad_small_launch(const float *rec, const _Float16 *fin, const float *ftgt, const uint32_t *dc,
+                             uint32_t mc, uint32_t clamp, const _Float16 *wf, const _Float16 *wtf,
+                             const DeviceState *st, int parity, float *partials, uint32_t pstride, uint32_t *overflow,
+                             uint32_t nblocks, hipStream_t s) {
+	const size_t smem = sizeof(SmallGradSmem<H>);
+	auto kern = fused_grad_small_kernel<H, ENC, LOSS, F16IN>;
+	static bool attr_set = false;
+	if (!attr_set) {
+		hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+		if (e != hipSuccess) return e;
+		attr_set = true;
+	}
+	kern<<<dim3(nblocks), dim3(256), smem, s>>>(rec, fin, ftgt, dc, mc, clamp, wf, wtf, st, parity, partials, pstride,
+	                                           overflow);
+	return hipGetLastError();
 }
 
-template <int H> hipError_t grad_h(const TrainArgs &a, uint32_t chunks, uint32_t nb, hipStream_t s) {
+template <int H, int ENC, int LOSS>
+hipError_t grad_loss(const TrainArgs &a, const FusedGradPlan &p, hipStream_t s) {
+	uint32_t *ovf = &a.state->overflow[a.parity];
+	if (p.records == kFusedSmallRecords)
+		return grad_small_launch<H, ENC, LOSS, false>(a.records, nullptr, nullptr, a.d_count, a.max_count, a.clamp,
+		                                              a.wf, a.wtf, a.state, a.parity, a.partials, a.pstride, ovf,
+		                                              p.blocks, s);
+	return grad_launch<H, ENC, LOSS, false>(a.records, nullptr, nullptr, a.d_count, a.max_count, a.clamp, p.chunks,
+	                                        a.w16, a.w16t, a.state, a.parity, a.partials, a.pstride, ovf, p.blocks,
+	                                        s);
+}
+
+template <int H, int ENC> hipError_t grad_enc(const TrainArgs &a, const FusedGradPlan &p, hipStream_t s) {
+	if (a.loss == NRC_LOSS_L2) return grad_loss<H, ENC, NRC_LOSS_L2>(a, p, s);
+	return grad_loss<H, ENC, NRC_LOSS_RELATIVE_L2_LUMINANCE>(a, p, s);
+}
+
+template <int H> hipError_t grad_h(const TrainArgs &a, const FusedGradPlan &p, hipStream_t s) {
 	switch (a.encoding) {
-	case NRC_ENCODING_TRI12: return grad_enc<H, NRC_ENCODING_TRI12>(a, chunks, nb, s);
-	case NRC_ENCODING_SINCOS6: return grad_enc<H, NRC_ENCODING_SINCOS6>(a, chunks, nb, s);
-	default: return grad_enc<H, NRC_ENCODING_UV_SINCOS16>(a, chunks, nb, s);
+	case NRC_ENCODING_TRI12: return grad_enc<H, NRC_ENCODING_TRI12>(a, p, s);
+	case NRC_ENCODING_SINCOS6: return grad_enc<H, NRC_ENCODING_SINCOS6>(a, p, s);
+	default: return grad_enc<H, NRC_ENCODING_UV_SINCOS16>(a, p, s);
 	}
 }
 
@@ -779,7 +1068,9 @@ hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t
 	return infer_launch<4, NRC_ENCODING_TRI12, true>(nullptr, inputs, nullptr, n, w16, out, 0, stream);
 }
 
-uint32_t fused_train_chunks(uint32_t n) {
+// Multi-chunk kernel: chunks per workgroup so that about one workgroup runs per
+// CU (VKNRC_TRAIN_CHUNKS overrides).
+static uint32_t fused_train_chunks(uint32_t n) {
 	static int forced = -1;
 	if (forced < 0) {
 		const char *e = getenv("VKNRC_TRAIN_CHUNKS");
@@ -787,28 +1078,63 @@ uint32_t fused_train_chunks(uint32_t n) {
 	}
 	if (forced > 0) return (uint32_t)(forced < (int)kFusedMaxChunks ? forced : (int)kFusedMaxChunks);
 	const uint32_t nchunks = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
-	const uint32_t target = (uint32_t)device_cu_count(); // workgroups per step: at most one per CU
+	const uint32_t target = (uint32_t)device_cu_count();
 	uint32_t c = target ? (nchunks + target - 1) / target : 1;
 	if (c < 1) c = 1;
 	if (c > kFusedMaxChunks) c = kFusedMaxChunks;
 	return c;
 }
 
-hipError_t launch_fused_gradient(const TrainArgs &a, uint32_t chunks, uint32_t *nparts, hipStream_t stream) {
-	const uint32_t n = a.max_count < a.clamp ? a.max_count : a.clamp;
+// Most workgroups of the single-chunk kernel before the multi-chunk kernel takes
+// over (every workgroup writes a partial row); VKNRC_SMALL_GRAD_BLOCKS overrides.
+static uint32_t small_grad_max_blocks() {
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("VKNRC_SMALL_GRAD_BLOCKS");
+		v = e ? atoi(e) : 2 * device_cu_count();
+		if (v < 0) v = 0;
+	}
+	return (uint32_t)v;
+}
+
+FusedGradPlan plan_fused_gradient(uint32_t n) {
+	FusedGradPlan p;
+	const uint32_t small = (n + kFusedSmallRecords - 1) / kFusedSmallRecords;
+	if (small <= small_grad_max_blocks()) {
+		p.records = kFusedSmallRecords;
+		p.chunks = 1;
+		p.blocks = small ? small : 1;
+		return p;
+	}
+	p.records = kFusedTrainSamples;
+	p.chunks = fused_train_chunks(n);
 	const uint32_t nchunks = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
-	uint32_t nb = (nchunks + chunks - 1) / chunks;
-	if (nb == 0) nb = 1;
-	*nparts = nb;
-	if (a.hidden == 5) return grad_h<5>(a, chunks, nb, stream);
-	return grad_h<4>(a, chunks, nb, stream);
+	p.blocks = (nchunks + p.chunks - 1) / p.chunks;
+	if (p.blocks == 0) p.blocks = 1;
+	return p;
+}
+
+hipError_t launch_fused_gradient(const TrainArgs &a, const FusedGradPlan &plan, hipStream_t stream) {
+	if (a.hidden == 5) return grad_h<5>(a, plan, stream);
+	return grad_h<4>(a, plan, stream);
 }
 
 hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,
-                                     const _Float16 *w16, const _Float16 *w16t, float *partials, uint32_t pstride,
-                                     uint32_t *nparts, hipStream_t stream) {
-	uint32_t nb = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
-	if (nb == 0) nb = 1;
+                                     const _Float16 *w16, const _Float16 *w16t, const _Float16 *wf,
+                                     const _Float16 *wtf, float *partials, uint32_t pstride, uint32_t *nparts,
+                                     hipStream_t stream) {
+	const FusedGradPlan p = plan_fused_gradient(n);
+	*nparts = p.blocks;
+	if (p.records == kFusedSmallRecords) {
+		if (hidden == 5)
+			return grad_small_launch<5, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(
+			    nullptr, inputs, targets, nullptr, n, n, wf, wtf, nullptr, 0, partials, pstride, nullptr, p.blocks,
+			    stream);
+		return grad_small_launch<4, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(
+		    nullptr, inputs, targets, nullptr, n, n, wf, wtf, nullptr, 0, partials, pstride, nullptr, p.blocks, stream);
+	}
+	// one 128-record chunk per workgroup
+	const uint32_t nb = (n + kFusedTrainSamples - 1) / kFusedTrainSamples;
 	*nparts = nb;
 	if (hidden == 5)
 		return grad_launch<5, NRC_ENCODING_TRI12, NRC_LOSS_L2, true>(nullptr, inputs, targets, nullptr, n, n, 1, w16,

@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
 		a.w16[i] = h;
 		const uint32_t hid = (uint32_t)a.hidden * 4096u;
 		a.w16t[i < hid ? ((i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)) : i] = h;
+		frag_scatter(i, h, a.wf, a.wtf);
 		if (a.write_use) { // nrc_optimize.comp:50-53
 			const float u = a.p.use_ema ? e.ema_weight : e.weight;
 			a.use32[i] = u;
@@ -159,6 +160,13 @@ __global__ __launch_bounds__(256) void transpose_hidden_kernel(const _Float16 *_
 	w16t[(i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)] = w16[i];
 }
 
+__global__ __launch_bounds__(256) void make_frags_kernel(const _Float16 *__restrict__ w16, _Float16 *__restrict__ wf,
+                                                         _Float16 *__restrict__ wtf, uint32_t n) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	frag_scatter(i, w16[i], wf, wtf);
+}
+
 __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 	nrc_optimizer_state s;
 	s.t = 0;
@@ -195,6 +203,12 @@ hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32
 hipError_t launch_transpose_hidden(int hidden, const _Float16 *w16, _Float16 *w16t, hipStream_t stream) {
 	const uint32_t n = (uint32_t)hidden * 4096u;
 	transpose_hidden_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(w16, w16t, n);
+	return hipGetLastError();
+}
+
+hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Float16 *wtf, hipStream_t stream) {
+	const uint32_t n = (uint32_t)weight_count(hidden);
+	make_frags_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(w16, wf, wtf, n);
 	return hipGetLastError();
 }
 

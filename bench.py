@@ -9,6 +9,12 @@ before the timed region.
 
 Prints one JSON line.  value = queries per second of whole frames (inference
 plus the 4 training steps), measured between two device synchronisations.
+
+The line also carries the 16-cache workload (`group16`): 16 independent caches
+with distinct seeded weights and light sets, per-frame query counts
+log-uniform in [65,536, 2,073,600] and training batches uniform in
+{4096, 8192, 16384}, timed as sequential per-cache frames, as one grouped frame
+and as a replayed HIP graph of the grouped frame.
 """
 from __future__ import annotations
 
@@ -71,6 +77,49 @@ def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> d
             "train_sample": f"{treps} x {BATCH}-record forward+backward (no optimizer)"}
 
 
+def group_workload(torch, v, wl, q_host, frames: int) -> dict:
+    """16 caches: sequential nrc_frame calls vs one nrc_group_frame vs a replayed graph."""
+    rng = np.random.default_rng(wl.config_seed(3))
+    n = 16
+    nq = np.exp(rng.uniform(np.log(65536), np.log(QUERIES), n)).astype(np.int64)
+    nb = rng.choice([4096, 8192, 16384], n)
+    dq = torch.from_numpy(q_host).to("cuda")
+    entries = []
+    for i in range(n):
+        st = v.NRCState(v.NRCConfig(seed=wl.config_seed(3, 100 + i), encoding=v.ENCODING_SINCOS6,
+                                    loss=v.LOSS_RELATIVE_L2_LUMINANCE, loss_scale=128.0, dynamic_loss_scale=True))
+        recs = wl.make_train_records(4 * int(nb[i]), rng, wl.GaussianLights(wl.config_seed(3, 200 + i)))
+        entries.append(dict(cache=st, queries=dq[:int(nq[i])], max_queries=int(nq[i]),
+                            rgb=torch.zeros((int(nq[i]), 3), dtype=torch.float32, device="cuda"),
+                            batches=torch.from_numpy(recs).to("cuda"), max_batch_records=int(nb[i])))
+
+    def sequential():
+        for e in entries:
+            e["cache"].frame(e["queries"], e["rgb"], e["batches"], max_queries=e["max_queries"],
+                             max_batch_records=e["max_batch_records"])
+
+    graph = v.GroupGraph(entries)
+    modes = {"sequential": sequential, "grouped": lambda: v.group_frame(entries), "graph": graph.launch}
+    out = {}
+    for name, fn in modes.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) * 1e3 / frames
+    graph.close()
+    total_q = int(nq.sum())
+    total_t = int(4 * nb.sum())
+    return {"caches": n, "queries_per_frame": total_q, "train_samples_per_frame": total_t, "frames": frames,
+            "sequential_ms_per_frame": out["sequential"], "grouped_ms_per_frame": out["grouped"],
+            "graph_ms_per_frame": out["graph"], "graph_queries_per_s": total_q / (out["graph"] / 1e3),
+            "grouped_vs_sequential": out["sequential"] / out["grouped"],
+            "graph_vs_sequential": out["sequential"] / out["graph"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +128,7 @@ def main() -> int:
     ap.add_argument("--plain-reps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--group-frames", type=int, default=30, help="timed frames of the 16-cache workload (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run one elementwise copy (torch.neg) of 256 MiB fp32 -> 256 MiB, the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -182,6 +232,9 @@ def main() -> int:
                      "traffic": None,
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
     }
+    if args.group_frames > 0:
+        log("16-cache workload ...")
+        res["group16"] = group_workload(torch, v, wl, q, args.group_frames)
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

@@ -193,6 +193,45 @@ nrc_status nrc_mlp_forward_f16(uint32_t hidden_layers, const uint16_t *d_weights
 nrc_status nrc_mlp_gradient_f16(uint32_t hidden_layers, const uint16_t *d_weights_f16, const uint16_t *d_inputs_f16,
                                 const float *d_targets, uint32_t n, float *d_dw, void *stream);
 
+/*
+ * Many caches at once.  The reference drives one cache per renderer
+ * (NRCRenderGraph.cpp:46-80); these run the same per-cache passes for a group
+ * of independent caches in shared launches: every cache's result is the one
+ * nrc_infer / nrc_train / nrc_frame would give it alone (bit-identical).
+ * Caches in one call must be distinct and on the current device; they may
+ * differ in hidden layers, encoding and loss (they are launched by kind).
+ */
+typedef struct nrc_group_item {
+	nrc_cache *cache;
+	const nrc_query *d_queries;          /* inference input, NULL with max_queries == 0 */
+	const uint32_t *d_query_count;       /* device count or NULL (= max_queries) */
+	uint32_t max_queries;
+	float *d_rgb;                        /* [max_queries][3] */
+	const nrc_train_record *d_batches;   /* NRC_TRAIN_BATCH_COUNT batches of max_batch_records */
+	const uint32_t *d_batch_counts;      /* NRC_TRAIN_BATCH_COUNT device counts or NULL */
+	uint32_t max_batch_records;          /* 0: no training for this cache */
+} nrc_group_item;
+
+/* nrc_infer for every item. */
+nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream);
+/* nrc_train on batch `batch` (0 .. NRC_TRAIN_BATCH_COUNT-1) of every item with training. */
+nrc_status nrc_group_train(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
+                           uint32_t flags, void *stream);
+/* nrc_frame for every item: one grouped inference, then NRC_TRAIN_BATCH_COUNT grouped steps. */
+nrc_status nrc_group_frame(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream);
+
+/*
+ * A group frame captured once as a HIP graph and replayed per frame (device
+ * counts may change between replays; buffers, max sizes and caches may not).
+ * A frame takes an even number of steps, so every replay starts from the same
+ * optimizer-state buffer; a replay after any other train call on one of the
+ * caches is refused with NRC_ERROR_INVALID_ARGUMENT.
+ */
+typedef struct nrc_graph nrc_graph;
+nrc_status nrc_group_frame_graph(const nrc_group_item *items, uint32_t n_items, nrc_path path, nrc_graph **out_graph);
+nrc_status nrc_graph_launch(nrc_graph *graph, void *stream);
+void nrc_graph_destroy(nrc_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,18 @@ def test_null_arguments_return_errors_not_aborts():
 def test_struct_layouts_match_the_header():
     assert C.sizeof(v.nrc.OptimizerState) == 40
     assert C.sizeof(v.nrc.CacheDesc) == 64
+    assert C.sizeof(v.nrc.GroupItem) == 64
+
+
+def test_group_calls_validate_before_touching_a_device():
+    L = v.load_library()
+    item = v.nrc.GroupItem()  # NULL cache
+    assert L.nrc_group_infer(C.byref(item), 1, v.PATH_FUSED_F16, None) != 0
+    assert b"NULL cache" in L.nrc_last_error() or b"hip" in L.nrc_last_error().lower()
+    g = C.c_void_p()
+    assert L.nrc_group_frame_graph(None, 0, v.PATH_FUSED_F16, None) != 0
+    assert L.nrc_graph_launch(None, None) != 0
+    L.nrc_graph_destroy(None)
 
 
 def test_workload_generators():

@@ -241,6 +241,22 @@ def test_fused_train_step_matches_oracle(o64, hidden, enc, loss):
     assert s["state"]["t"] == 1
 
 
+@pytest.mark.parametrize("n", [40000, 65536])
+def test_fused_multi_chunk_train_step_matches_oracle(o64, n):
+    """Batches above 64 records per CU: workgroups walk several chunks and sum dW in registers."""
+    rng = np.random.default_rng(90 + n % 7)
+    st, w = make_cache(rng, train_batch_size=65536, encoding=v.ENCODING_SINCOS6)
+    r = train_records(n, 91, v.ENCODING_SINCOS6)
+    st.train(dev(r), path=v.PATH_FUSED_F16)
+    s = st.get_state()
+    g = s["m"].astype(np.float64) / (1.0 - np.float32(0.9))
+    G, lsum = o64.gradient(f16(w), 5, r, v.ENCODING_SINCOS6, v.LOSS_RELATIVE_L2_LUMINANCE, round_f16=True)
+    assert cosine(g, G / n) > GRAD_COSINE_MIN, cosine(g, G / n)
+    assert rel_err(g, G / n) < 5e-2
+    assert abs(s["state"]["last_loss"] - lsum / n) < 1e-2 * abs(lsum / n)
+    assert s["state"]["last_count"] == n
+
+
 def test_train_is_deterministic():
     rng = np.random.default_rng(5)
     w = rng.normal(0, 0.17, v.weight_count(5)).astype(np.float32)
@@ -314,19 +330,36 @@ def test_nonfinite_gradient_semantics():
     st.train(dev(bad), path=v.PATH_PLAIN_F32)
     s = st.get_state()
     assert s["state"]["t"] == 1 and np.isfinite(s["weight"]).all()
-    # dynamic scale: an overflowing fp16 gradient skips the step and halves the scale
+    # dynamic scale: a non-finite gradient skips the step and halves the scale
     st2, w2 = make_cache(rng, loss_scale=128.0, dynamic_loss_scale=True)
     huge = r.copy()
-    huge[3, 14:17] = 3e5
+    huge[3, 14:17] = 1e38   # the FP32 output gradient itself overflows
     st2.train(dev(huge), path=v.PATH_FUSED_F16)
     s2 = st2.get_state()
     assert s2["state"]["t"] == 0 and s2["state"]["skipped_steps"] == 1 and s2["state"]["loss_scale"] == 64.0
     assert np.array_equal(s2["weight"], w2)
     mild = r.copy()
-    mild[:, 14:17] = np.clip(mild[:, 14:17], 0.0, 1.0)   # |g| <= 2 * 64 / 0.01: no fp16 overflow at scale 64
+    mild[:, 14:17] = np.clip(mild[:, 14:17], 0.0, 1.0)
     st2.train(dev(mild), path=v.PATH_FUSED_F16)
     s3 = st2.get_state()
     assert s3["state"]["t"] == 1 and s3["state"]["loss_scale"] == 64.0 and s3["state"]["clean_steps"] == 1
+
+
+def test_large_finite_gradients_are_scaled_not_skipped(o64):
+    """An output gradient beyond FP16 range (2 * 128 * 3e5 / 0.01 here) puts the
+    workgroup's deltas under a power-of-two scale instead of overflowing: the fused
+    step is taken and matches the oracle like any other."""
+    rng = np.random.default_rng(15)
+    r = train_records(3000, 16)
+    r[3, 14:17] = 3e5
+    r[2500, 14:17] = 1e6
+    st, w = make_cache(rng, loss_scale=128.0, dynamic_loss_scale=True)
+    st.train(dev(r), path=v.PATH_FUSED_F16)
+    s = st.get_state()
+    assert s["state"]["t"] == 1 and s["state"]["skipped_steps"] == 0 and s["state"]["loss_scale"] == 128.0
+    g = s["m"].astype(np.float64) / (1.0 - np.float32(0.9))
+    G, _ = o64.gradient(f16(w), 5, r, v.ENCODING_TRI12, v.LOSS_RELATIVE_L2_LUMINANCE, round_f16=True)
+    assert cosine(g, G / 3000) > GRAD_COSINE_MIN, cosine(g, G / 3000)
 
 
 def test_frame_equals_infer_then_four_train_steps():

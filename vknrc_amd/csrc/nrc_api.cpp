@@ -18,8 +18,9 @@ struct nrc_cache {
 	int device = 0;
 	uint32_t wc = 0;
 	nrc_optimizer_entry *entries = nullptr;
-	_Float16 *w16 = nullptr, *w16t = nullptr, *use16 = nullptr;
+	_Float16 *w16 = nullptr, *use16 = nullptr;
 	_Float16 *wf = nullptr, *wtf = nullptr; // fragment-major copies for the fused gradient
+	_Float16 *usef = nullptr;               // fragment-major use-weights for the fused inference
 	float *w32 = nullptr, *use32 = nullptr;
 	DeviceState *state = nullptr;
 	float *partials = nullptr;
@@ -101,25 +102,24 @@ void host_init_weights(const nrc_cache_desc *d, uint64_t seed, float *w) {
 // Uploads entries + the derived fp16/fp32 weight copies (ResetMLPBuffers, VkNRCState.cpp:46-88).
 nrc_status upload_entries(nrc_cache *c, const nrc_optimizer_entry *entries) {
 	const int H = (int)c->desc.hidden_layers;
-	std::vector<_Float16> w16(c->wc), w16t(c->wc), u16(c->wc);
+	std::vector<_Float16> w16(c->wc), u16(c->wc);
 	std::vector<_Float16> wf(nrc::fwd_frag_halves(H), (_Float16)0.0f), wtf(nrc::bwd_frag_halves(H), (_Float16)0.0f);
+	std::vector<_Float16> usef(nrc::fwd_frag_halves(H), (_Float16)0.0f);
 	std::vector<float> w32(c->wc), u32(c->wc);
-	const uint32_t hid = c->desc.hidden_layers * 4096u;
 	for (uint32_t i = 0; i < c->wc; ++i) {
 		w32[i] = entries[i].weight;
 		u32[i] = c->desc.use_ema_weights ? entries[i].ema_weight : entries[i].weight;
 		w16[i] = (_Float16)w32[i];
 		u16[i] = (_Float16)u32[i];
-		// transposed hidden layers [l][in][out] for the fused backward
-		w16t[i < hid ? ((i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)) : i] = w16[i];
 		nrc::frag_scatter(i, w16[i], wf.data(), wtf.data());
+		nrc::frag_scatter(i, u16[i], usef.data(), (_Float16 *)nullptr);
 	}
 	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
 	NRC_HIP(hipMemcpy(c->entries, entries, sizeof(nrc_optimizer_entry) * c->wc, hipMemcpyHostToDevice), "upload entries");
 	NRC_HIP(hipMemcpy(c->w16, w16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
-	NRC_HIP(hipMemcpy(c->w16t, w16t.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->wf, wf.data(), 2 * wf.size(), hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->wtf, wtf.data(), 2 * wtf.size(), hipMemcpyHostToDevice), "upload weights");
+	NRC_HIP(hipMemcpy(c->usef, usef.data(), 2 * usef.size(), hipMemcpyHostToDevice), "upload use weights");
 	NRC_HIP(hipMemcpy(c->use16, u16.data(), 2 * c->wc, hipMemcpyHostToDevice), "upload use weights");
 	NRC_HIP(hipMemcpy(c->w32, w32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload weights");
 	NRC_HIP(hipMemcpy(c->use32, u32.data(), 4 * c->wc, hipMemcpyHostToDevice), "upload use weights");
@@ -143,7 +143,7 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 void free_cache(nrc_cache *c) {
 	if (!c) return;
 	(void)hipSetDevice(c->device);
-	void *ptrs[] = {c->entries, c->w16, c->w16t, c->wf, c->wtf, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
+	void *ptrs[] = {c->entries, c->w16, c->wf, c->wtf, c->usef, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	delete c;
@@ -170,6 +170,67 @@ nrc_status ensure_plain_scratch(nrc_cache *c) {
 	NRC_HIP(hipMalloc(&c->deltas, slab * sizeof(float)), "plain scratch");
 	NRC_HIP(hipMalloc(&c->sample_loss, (size_t)ldn * sizeof(float)), "plain scratch");
 	c->ldn = ldn;
+	return NRC_OK;
+}
+
+
+// Arguments of one training step on `c` (the plain-path scratch is added by the caller).
+nrc::TrainArgs train_args(nrc_cache *c, const nrc_train_record *d_records, const uint32_t *d_count,
+                          uint32_t max_count) {
+	nrc::TrainArgs a{};
+	a.hidden = (int)c->desc.hidden_layers;
+	a.encoding = (int)c->desc.encoding;
+	a.loss = (int)c->desc.loss;
+	a.records = reinterpret_cast<const float *>(d_records);
+	a.d_count = d_count;
+	a.max_count = max_count;
+	a.clamp = c->desc.train_batch_size;
+	a.w32 = c->w32;
+	a.wf = c->wf;
+	a.wtf = c->wtf;
+	a.state = c->state;
+	a.parity = c->parity;
+	a.partials = c->partials;
+	a.pstride = c->pstride;
+	return a;
+}
+
+nrc::AdamArgs adam_args(nrc_cache *c, const uint32_t *d_count, uint32_t max_count, uint32_t nparts, uint32_t flags) {
+	nrc::AdamArgs o{};
+	o.partials = c->partials;
+	o.nparts = nparts;
+	o.pstride = c->pstride;
+	o.d_count = d_count;
+	o.max_count = max_count;
+	o.clamp = c->desc.train_batch_size;
+	o.state = c->state;
+	o.parity = c->parity;
+	o.entries = c->entries;
+	o.w16 = c->w16;
+	o.wf = c->wf;
+	o.wtf = c->wtf;
+	o.use16 = c->use16;
+	o.usef = c->usef;
+	o.w32 = c->w32;
+	o.use32 = c->use32;
+	o.write_use = (flags & NRC_TRAIN_WRITE_USE_WEIGHTS) ? 1 : 0;
+	o.weight_count = c->wc;
+	o.hidden = (int)c->desc.hidden_layers;
+	o.p = adam_params(c);
+	return o;
+}
+
+// Argument checks of one training step, including the partial rows its plan
+// writes, before anything is launched.
+nrc_status check_train(nrc_cache *c, const nrc_train_record *d_records, uint32_t max_count, nrc_path path) {
+	if (!d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL records");
+	if (max_count > NRC_MAX_TRAIN_BATCH_SIZE) return fail(NRC_ERROR_CAPACITY, "max_count > NRC_MAX_TRAIN_BATCH_SIZE");
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	const uint32_t clamp = c->desc.train_batch_size;
+	const uint32_t n = max_count < clamp ? max_count : clamp;
+	const uint32_t need = path == NRC_PATH_PLAIN_F32 ? (n + nrc::kPlainChunk - 1) / nrc::kPlainChunk
+	                                                 : nrc::plan_fused_gradient(n).blocks;
+	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
 	return NRC_OK;
 }
 
@@ -237,9 +298,9 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	hipError_t e = hipGetDevice(&c->device);
 	if (e == hipSuccess) e = hipMalloc(&c->entries, sizeof(nrc_optimizer_entry) * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w16, 2 * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->w16t, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->wf, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
 	if (e == hipSuccess) e = hipMalloc(&c->wtf, 2 * (size_t)nrc::bwd_frag_halves((int)desc->hidden_layers));
+	if (e == hipSuccess) e = hipMalloc(&c->usef, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
 	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
@@ -250,6 +311,10 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 		return hip_fail(e, "nrc_cache_create");
 	}
 	nrc_status s = reset_cache(c, desc->seed);
+	if (s == NRC_OK) {
+		e = nrc::prepare_fused((int)desc->hidden_layers, (int)desc->encoding, (int)desc->loss);
+		if (e != hipSuccess) s = hip_fail(e, "nrc_cache_create kernel set-up");
+	}
 	if (s != NRC_OK) {
 		free_cache(c);
 		return s;
@@ -288,7 +353,14 @@ nrc_status nrc_infer(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d
 	hipError_t e;
 	switch (path) {
 	case NRC_PATH_PLAIN_F32: e = nrc::launch_plain_infer(H, enc, q, d_count, max_count, c->use32, d_rgb, s); break;
-	case NRC_PATH_FUSED_F16: e = nrc::launch_fused_infer(H, enc, q, d_count, max_count, c->use16, d_rgb, s); break;
+	case NRC_PATH_FUSED_F16: {
+		nrc::InferGroup grp{};
+		grp.n_items = 1;
+		grp.clamp0 = 1;
+		grp.item[0] = nrc::InferItem{q, d_count, d_rgb, c->usef, max_count, 0};
+		e = nrc::launch_fused_infer_group(H, enc, grp, s);
+		break;
+	}
 	default: return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_infer launch");
@@ -299,36 +371,15 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
                      nrc_path path, uint32_t flags, void *stream) {
 	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
 	if (max_count == 0) return NRC_OK;
-	if (!d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL records");
-	if (max_count > NRC_MAX_TRAIN_BATCH_SIZE) return fail(NRC_ERROR_CAPACITY, "max_count > NRC_MAX_TRAIN_BATCH_SIZE");
-	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	nrc_status st = check_train(c, d_records, max_count, path);
+	if (st != NRC_OK) return st;
 	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-	nrc::TrainArgs a{};
-	a.hidden = (int)c->desc.hidden_layers;
-	a.encoding = (int)c->desc.encoding;
-	a.loss = (int)c->desc.loss;
-	a.records = reinterpret_cast<const float *>(d_records);
-	a.d_count = d_count;
-	a.max_count = max_count;
-	a.clamp = c->desc.train_batch_size;
-	a.w32 = c->w32;
-	a.w16 = c->w16;
-	a.w16t = c->w16t;
-	a.wf = c->wf;
-	a.wtf = c->wtf;
-	a.state = c->state;
-	a.parity = c->parity;
-	a.partials = c->partials;
-	a.pstride = c->pstride;
-	// partial-gradient slots the launch will write, checked before anything runs
+	nrc::TrainArgs a = train_args(c, d_records, d_count, max_count);
 	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
-	const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
-	const uint32_t need = path == NRC_PATH_PLAIN_F32 ? (n + nrc::kPlainChunk - 1) / nrc::kPlainChunk : plan.blocks;
-	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
 	uint32_t nparts = 0;
 	hipError_t e;
 	if (path == NRC_PATH_PLAIN_F32) {
-		nrc_status st = ensure_plain_scratch(c);
+		st = ensure_plain_scratch(c);
 		if (st != NRC_OK) return st;
 		a.acts = c->acts;
 		a.deltas = c->deltas;
@@ -336,32 +387,12 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 		a.ldn = c->ldn;
 		e = nrc::launch_plain_gradient(a, &nparts, s);
 	} else {
+		const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
 		e = nrc::launch_fused_gradient(a, plan, s);
 		nparts = plan.blocks;
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
-	nrc::AdamArgs o{};
-	o.partials = c->partials;
-	o.nparts = nparts;
-	o.pstride = c->pstride;
-	o.d_count = d_count;
-	o.max_count = max_count;
-	o.clamp = c->desc.train_batch_size;
-	o.state = c->state;
-	o.parity = c->parity;
-	o.entries = c->entries;
-	o.w16 = c->w16;
-	o.w16t = c->w16t;
-	o.wf = c->wf;
-	o.wtf = c->wtf;
-	o.use16 = c->use16;
-	o.w32 = c->w32;
-	o.use32 = c->use32;
-	o.write_use = (flags & NRC_TRAIN_WRITE_USE_WEIGHTS) ? 1 : 0;
-	o.weight_count = c->wc;
-	o.hidden = (int)c->desc.hidden_layers;
-	o.p = adam_params(c);
-	e = nrc::launch_adam(o, s);
+	e = nrc::launch_adam(adam_args(c, d_count, max_count, nparts, flags), s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
 	c->parity ^= 1; // the optimizer wrote the other state buffer
 	return NRC_OK;
@@ -437,10 +468,18 @@ nrc_status nrc_mlp_forward_f16(uint32_t hidden, const uint16_t *d_w, const uint1
 	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");
 	if (n == 0) return NRC_OK;
 	if (!d_w || !d_in || !d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
-	hipError_t e = nrc::launch_fused_forward_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), n,
-	                                             reinterpret_cast<const _Float16 *>(d_w), d_out,
-	                                             reinterpret_cast<hipStream_t>(stream));
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	const size_t nf = nrc::fwd_frag_halves((int)hidden);
+	_Float16 *wf = nullptr;
+	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wf), 2 * nf, s), "forward scratch");
+	hipError_t e = hipMemsetAsync(wf, 0, 2 * nf, s);
+	if (e == hipSuccess)
+		e = nrc::launch_make_frags((int)hidden, reinterpret_cast<const _Float16 *>(d_w), wf, nullptr, s);
+	if (e == hipSuccess)
+		e = nrc::launch_fused_forward_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), n, wf, d_out, s);
+	hipError_t e2 = hipFreeAsync(wf, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_forward_f16");
+	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_forward_f16 free");
 	return NRC_OK;
 }
 
@@ -456,27 +495,229 @@ nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint
 	const uint32_t nb = n == 0 ? 1 : (n + nrc::kFusedSmallRecords - 1) / nrc::kFusedSmallRecords;
 	const size_t nf = nrc::fwd_frag_halves((int)hidden), nt = nrc::bwd_frag_halves((int)hidden);
 	float *partials = nullptr;
-	_Float16 *wt = nullptr, *wf = nullptr, *wtf = nullptr;
+	_Float16 *wf = nullptr, *wtf = nullptr;
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s), "gradient scratch");
-	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wt), 2 * (size_t)wc, s), "gradient scratch");
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wf), 2 * nf, s), "gradient scratch");
 	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wtf), 2 * nt, s), "gradient scratch");
 	uint32_t nparts = 0;
 	const _Float16 *w = reinterpret_cast<const _Float16 *>(d_w);
-	hipError_t e = nrc::launch_transpose_hidden((int)hidden, w, wt, s);
-	if (e == hipSuccess) e = hipMemsetAsync(wf, 0, 2 * nf, s);
+	hipError_t e = hipMemsetAsync(wf, 0, 2 * nf, s);
 	if (e == hipSuccess) e = hipMemsetAsync(wtf, 0, 2 * nt, s);
 	if (e == hipSuccess) e = nrc::launch_make_frags((int)hidden, w, wf, wtf, s);
 	if (e == hipSuccess)
-		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, w, wt,
-		                                   wf, wtf, partials, pstride, &nparts, s);
+		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, wf, wtf,
+		                                   partials, pstride, &nparts, s);
 	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
 	hipError_t e2 = hipFreeAsync(partials, s);
-	if (e2 == hipSuccess) e2 = hipFreeAsync(wt, s);
 	if (e2 == hipSuccess) e2 = hipFreeAsync(wf, s);
 	if (e2 == hipSuccess) e2 = hipFreeAsync(wtf, s);
 	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16");
 	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_gradient_f16 free");
+	return NRC_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------ groups
+struct nrc_graph {
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	std::vector<std::pair<nrc_cache *, int>> parity; // each cache's step parity at capture
+};
+
+namespace {
+
+nrc_status check_group(const nrc_group_item *items, uint32_t n, bool distinct) {
+	if (n && !items) return fail(NRC_ERROR_INVALID_ARGUMENT, "items is NULL");
+	int dev = 0;
+	NRC_HIP(hipGetDevice(&dev), "hipGetDevice");
+	for (uint32_t i = 0; i < n; ++i) {
+		const nrc_cache *c = items[i].cache;
+		if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "group item has a NULL cache");
+		if (c->device != dev) return fail(NRC_ERROR_INVALID_ARGUMENT, "group caches must be on the current device");
+		if (distinct)
+			for (uint32_t j = 0; j < i; ++j)
+				if (items[j].cache == c) return fail(NRC_ERROR_INVALID_ARGUMENT, "a cache appears twice in one group");
+	}
+	return NRC_OK;
+}
+
+// Fused launches take items of one kind (same hidden layers, encoding and, for
+// training, loss) and at most kGroupMax of them: `sel` lists the item indices of
+// each launch.
+std::vector<std::vector<uint32_t>> group_launches(const nrc_group_item *items, const std::vector<uint32_t> &use,
+                                                  bool by_loss) {
+	std::vector<std::vector<uint32_t>> out;
+	std::vector<bool> taken(use.size(), false);
+	for (size_t a = 0; a < use.size(); ++a) {
+		if (taken[a]) continue;
+		const nrc_cache_desc &da = items[use[a]].cache->desc;
+		std::vector<uint32_t> cur;
+		for (size_t b = a; b < use.size(); ++b) {
+			const nrc_cache_desc &db = items[use[b]].cache->desc;
+			if (taken[b] || db.hidden_layers != da.hidden_layers || db.encoding != da.encoding ||
+			    (by_loss && db.loss != da.loss))
+				continue;
+			taken[b] = true;
+			cur.push_back(use[b]);
+			if (cur.size() == nrc::kGroupMax) {
+				out.push_back(cur);
+				cur.clear();
+			}
+		}
+		if (!cur.empty()) out.push_back(cur);
+	}
+	return out;
+}
+
+} // namespace
+
+extern "C" {
+
+nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
+	nrc_status st = check_group(items, n_items, false);
+	if (st != NRC_OK) return st;
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	std::vector<uint32_t> use;
+	for (uint32_t i = 0; i < n_items; ++i) {
+		if (items[i].max_queries == 0) continue;
+		if (!items[i].d_queries || !items[i].d_rgb) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+		use.push_back(i);
+	}
+	if (path == NRC_PATH_PLAIN_F32) {
+		for (uint32_t i : use) {
+			const nrc_group_item &it = items[i];
+			st = nrc_infer(it.cache, it.d_queries, it.d_query_count, it.max_queries, it.d_rgb, path, stream);
+			if (st != NRC_OK) return st;
+		}
+		return NRC_OK;
+	}
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	for (const auto &sel : group_launches(items, use, false)) {
+		nrc::InferGroup grp{};
+		grp.clamp0 = 1;
+		for (uint32_t i : sel) {
+			const nrc_group_item &it = items[i];
+			grp.item[grp.n_items++] = nrc::InferItem{reinterpret_cast<const float *>(it.d_queries), it.d_query_count,
+			                                         it.d_rgb, it.cache->usef, it.max_queries, 0};
+		}
+		const nrc_cache_desc &d = items[sel[0]].cache->desc;
+		hipError_t e = nrc::launch_fused_infer_group((int)d.hidden_layers, (int)d.encoding, grp, s);
+		if (e != hipSuccess) return hip_fail(e, "nrc_group_infer launch");
+	}
+	return NRC_OK;
+}
+
+nrc_status nrc_group_train(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
+                           uint32_t flags, void *stream) {
+	nrc_status st = check_group(items, n_items, true);
+	if (st != NRC_OK) return st;
+	if (batch >= NRC_TRAIN_BATCH_COUNT) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch >= NRC_TRAIN_BATCH_COUNT");
+	std::vector<uint32_t> use;
+	for (uint32_t i = 0; i < n_items; ++i) {
+		const nrc_group_item &it = items[i];
+		if (it.max_batch_records == 0) continue;
+		st = check_train(it.cache, it.d_batches, it.max_batch_records, path);
+		if (st != NRC_OK) return st;
+		use.push_back(i);
+	}
+	auto records = [&](const nrc_group_item &it) { return it.d_batches + (size_t)batch * it.max_batch_records; };
+	auto counts = [&](const nrc_group_item &it) { return it.d_batch_counts ? it.d_batch_counts + batch : nullptr; };
+	if (path == NRC_PATH_PLAIN_F32) {
+		for (uint32_t i : use) {
+			const nrc_group_item &it = items[i];
+			st = nrc_train(it.cache, records(it), counts(it), it.max_batch_records, path, flags, stream);
+			if (st != NRC_OK) return st;
+		}
+		return NRC_OK;
+	}
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	for (const auto &sel : group_launches(items, use, true)) {
+		nrc::GradGroup grad{};
+		nrc::AdamGroup adam{};
+		for (uint32_t i : sel) {
+			const nrc_group_item &it = items[i];
+			nrc_cache *c = it.cache;
+			const nrc::TrainArgs a = train_args(c, records(it), counts(it), it.max_batch_records);
+			const uint32_t n = it.max_batch_records < a.clamp ? it.max_batch_records : a.clamp;
+			const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+			grad.item[grad.n_items++] = nrc::grad_item(a, plan);
+			adam.item[adam.n_items++] = adam_args(c, counts(it), it.max_batch_records, plan.blocks, flags);
+		}
+		const nrc_cache_desc &d = items[sel[0]].cache->desc;
+		hipError_t e = nrc::launch_fused_gradient_group((int)d.hidden_layers, (int)d.encoding, (int)d.loss, grad, s);
+		if (e != hipSuccess) return hip_fail(e, "nrc_group_train gradient launch");
+		e = nrc::launch_adam_group(adam, s);
+		if (e != hipSuccess) return hip_fail(e, "nrc_group_train optimizer launch");
+		for (uint32_t i : sel) items[i].cache->parity ^= 1;
+	}
+	return NRC_OK;
+}
+
+nrc_status nrc_group_frame(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
+	nrc_status st = nrc_group_infer(items, n_items, path, stream);
+	for (uint32_t b = 0; st == NRC_OK && b < NRC_TRAIN_BATCH_COUNT; ++b)
+		st = nrc_group_train(items, n_items, b, path, b == NRC_TRAIN_BATCH_COUNT - 1 ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u,
+		                     stream);
+	return st;
+}
+
+void nrc_graph_destroy(nrc_graph *g) {
+	if (!g) return;
+	if (g->exec) (void)hipGraphExecDestroy(g->exec);
+	if (g->graph) (void)hipGraphDestroy(g->graph);
+	delete g;
+}
+
+nrc_status nrc_group_frame_graph(const nrc_group_item *items, uint32_t n_items, nrc_path path, nrc_graph **out) {
+	if (!out) return fail(NRC_ERROR_INVALID_ARGUMENT, "out_graph is NULL");
+	*out = nullptr;
+	nrc_status st = check_group(items, n_items, true);
+	if (st != NRC_OK) return st;
+	// nothing may allocate while the stream is captured
+	for (uint32_t i = 0; i < n_items; ++i)
+		if (path == NRC_PATH_PLAIN_F32 && items[i].max_batch_records) {
+			st = ensure_plain_scratch(items[i].cache);
+			if (st != NRC_OK) return st;
+		}
+	nrc_graph *g = new nrc_graph();
+	for (uint32_t i = 0; i < n_items; ++i) g->parity.emplace_back(items[i].cache, items[i].cache->parity);
+	hipStream_t s = nullptr;
+	hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		delete g;
+		return hip_fail(e, "nrc_group_frame_graph stream");
+	}
+	e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+	if (e == hipSuccess) {
+		st = nrc_group_frame(items, n_items, path, s);
+		hipError_t e2 = hipStreamEndCapture(s, &g->graph);
+		if (e2 != hipSuccess && st == NRC_OK) st = hip_fail(e2, "nrc_group_frame_graph capture");
+	} else {
+		st = hip_fail(e, "nrc_group_frame_graph begin capture");
+	}
+	(void)hipStreamDestroy(s);
+	// capturing ran no step: the parities are back where they were only if every
+	// cache took an even number of steps, which a frame does
+	for (auto &pc : g->parity) pc.first->parity = pc.second;
+	if (st == NRC_OK) {
+		e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+		if (e != hipSuccess) st = hip_fail(e, "nrc_group_frame_graph instantiate");
+	}
+	if (st != NRC_OK) {
+		nrc_graph_destroy(g);
+		return st;
+	}
+	*out = g;
+	return NRC_OK;
+}
+
+nrc_status nrc_graph_launch(nrc_graph *g, void *stream) {
+	if (!g || !g->exec) return fail(NRC_ERROR_INVALID_ARGUMENT, "graph is NULL");
+	for (const auto &pc : g->parity)
+		if (pc.first->parity != pc.second)
+			return fail(NRC_ERROR_INVALID_ARGUMENT, "a cache of the graph took steps outside it since capture");
+	NRC_HIP(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)), "nrc_graph_launch");
 	return NRC_OK;
 }
 

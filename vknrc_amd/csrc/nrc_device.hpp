@@ -33,11 +33,11 @@ __host__ __device__ inline uint32_t frag_pos(uint32_t l, uint32_t row, uint32_t 
 	return ((((l * 4u + (row >> 4)) * 2u + c) * 64u) + g * 16u + (row & 15u)) * 8u + j;
 }
 
-// Scatters weight i (row-major layer order) into both fragment copies.
+// Scatters weight i (row-major layer order) into the fragment copies (wtf may be null).
 template <typename T> __host__ __device__ inline void frag_scatter(uint32_t i, T h, T *wf, T *wtf) {
 	const uint32_t l = i >> 12, row = (i >> 6) & 63u, col = i & 63u;
 	wf[frag_pos(l, row, col)] = h;
-	if (l >= 1) wtf[frag_pos(l - 1, col, row)] = h;
+	if (wtf && l >= 1) wtf[frag_pos(l - 1, col, row)] = h;
 }
 
 // Device-resident optimizer state, double-buffered by a parity the host flips

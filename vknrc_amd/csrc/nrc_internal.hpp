@@ -11,19 +11,53 @@ namespace nrc {
 
 // Samples per chunk of the plain weight-gradient reduction.
 constexpr uint32_t kPlainChunk = 512;
-// Records per chunk of the multi-chunk fused gradient kernel (8 waves, one
-// 16-record N-tile each).
-constexpr uint32_t kFusedTrainSamples = 128;
-// Most chunks one multi-chunk fused-gradient workgroup walks.
-constexpr uint32_t kFusedMaxChunks = 16;
-// Records per workgroup of the single-chunk fused gradient kernel (4 waves, one
-// 16-record N-tile each, weights resident in registers).
+// Records per chunk of the fused gradient kernel (4 waves, one 16-record N-tile each).
 constexpr uint32_t kFusedSmallRecords = 64;
+// Most chunks one fused-gradient workgroup walks.
+constexpr uint32_t kFusedMaxChunks = 16;
+
+// Most caches one grouped launch takes (the tables travel as kernel arguments).
+constexpr uint32_t kGroupMax = 16;
+
+// One cache's share of a grouped fused inference.
+struct InferItem {
+	const float *queries;    // [max_count][14]
+	const uint32_t *d_count; // device count or null (= max_count)
+	float *out;              // [max_count][3]
+	const _Float16 *wf;      // fragment-major use-weights
+	uint32_t max_count;
+	uint32_t tile_begin; // first 64-record tile of this cache in the group (set by the launcher)
+};
+struct InferGroup {
+	InferItem item[kGroupMax];
+	const _Float16 *fin; // pre-encoded fp16 inputs [n][64] (single-item test entry point only)
+	uint32_t n_items, total_tiles;
+	int clamp0; // max(pred, 0) as nrc_inference.comp:48
+};
+
+// One cache's share of a grouped fused gradient.
+struct GradItem {
+	const float *records; // [max_count][17]
+	const uint32_t *d_count;
+	const _Float16 *wf, *wtf; // fragment-major forward / backward weights
+	const DeviceState *state; // loss scale source (null: scale 1)
+	float *partials;          // this cache's partial rows [blocks][pstride]
+	uint32_t *overflow;       // non-finite flag of this step (null: none)
+	uint32_t max_count, clamp, chunks, blocks, pstride;
+	uint32_t block_begin; // first workgroup of this cache in the group (set by the launcher)
+	int parity;
+};
+struct GradGroup {
+	GradItem item[kGroupMax];
+	const _Float16 *fin; // pre-encoded fp16 inputs / targets (single-item test entry point only)
+	const float *ftgt;
+	uint32_t n_items, total_blocks;
+};
 
 // How a fused gradient launch covers n records: `blocks` workgroups, each
-// writing one partial-gradient row, walking `chunks` chunks of `records`.
+// writing one partial-gradient row, walking `chunks` 64-record chunks.
 struct FusedGradPlan {
-	uint32_t records, chunks, blocks;
+	uint32_t chunks, blocks;
 };
 
 struct TrainArgs {
@@ -35,8 +69,6 @@ struct TrainArgs {
 	uint32_t max_count;
 	uint32_t clamp; // train batch size
 	const float *w32;
-	const _Float16 *w16;
-	const _Float16 *w16t; // hidden layers transposed, [l][in][out]
 	const _Float16 *wf, *wtf; // fragment-major forward / backward copies
 	DeviceState *state;
 	int parity;      // state buffer this step reads
@@ -55,12 +87,17 @@ struct AdamArgs {
 	DeviceState *state;
 	int parity;
 	nrc_optimizer_entry *entries;
-	_Float16 *w16, *w16t, *wf, *wtf, *use16;
+	_Float16 *w16, *wf, *wtf, *use16, *usef;
 	float *w32, *use32;
 	int write_use;
 	int hidden;
 	uint32_t weight_count;
+	uint32_t block_begin; // first block of this cache in a grouped launch (set by the launcher)
 	AdamParams p;
+};
+struct AdamGroup {
+	AdamArgs item[kGroupMax];
+	uint32_t n_items;
 };
 
 // nrc_plain.hip
@@ -73,29 +110,40 @@ hipError_t launch_sph_encode(const float *dirs, uint32_t n, float *uv, hipStream
 
 // nrc_optim.hip
 hipError_t launch_adam(const AdamArgs &a, hipStream_t stream);
+// One launch for every cache of the group; fills block_begin.
+hipError_t launch_adam_group(AdamGroup &grp, hipStream_t stream);
 hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32_t pstride, uint32_t n, float *out,
                                   hipStream_t stream);
 hipError_t launch_init_state(DeviceState *st, float loss_scale, hipStream_t stream);
-// w16t[l][k][m] = w16[l][m][k] for the hidden layers
-hipError_t launch_transpose_hidden(int hidden, const _Float16 *w16, _Float16 *w16t, hipStream_t stream);
 // Scatters row-major fp16 weights into the fragment-major copies (whose padding
-// must already be zero).
+// must already be zero); wtf may be null.
 hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Float16 *wtf, hipStream_t stream);
 
 // nrc_fused.hip
-hipError_t launch_fused_infer(int hidden, int encoding, const float *queries, const uint32_t *d_count,
-                              uint32_t max_count, const _Float16 *w16, float *rgb, hipStream_t stream);
-hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *w16, float *out,
+// All items share (hidden, encoding); fills tile_begin and total_tiles.
+hipError_t launch_fused_infer_group(int hidden, int encoding, InferGroup &grp, hipStream_t stream);
+// Forward over n pre-encoded fp16 input rows with fragment-major weights wf (no clamp).
+hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *wf, float *out,
                                     hipStream_t stream);
 // Workgroups and chunking of the fused gradient for a step of n records.
 FusedGradPlan plan_fused_gradient(uint32_t n);
 hipError_t launch_fused_gradient(const TrainArgs &a, const FusedGradPlan &plan, hipStream_t stream);
+// One launch over a group sharing (hidden, encoding, loss); fills block_begin and
+// total_blocks.
+hipError_t launch_fused_gradient_group(int hidden, int encoding, int loss, GradGroup &grp, hipStream_t stream);
+GradItem grad_item(const TrainArgs &a, const FusedGradPlan &plan);
 // Gradient of the L2 loss over n fp16 input rows; wf/wtf are the fragment-major copies.
 hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const float *targets, uint32_t n,
-                                     const _Float16 *w16, const _Float16 *w16t, const _Float16 *wf,
-                                     const _Float16 *wtf, float *partials, uint32_t pstride, uint32_t *nparts,
-                                     hipStream_t stream);
+                                     const _Float16 *wf, const _Float16 *wtf, float *partials, uint32_t pstride,
+                                     uint32_t *nparts, hipStream_t stream);
 
 int device_cu_count();
+// One-time launch set-up of the fused kernels a cache of this kind uses (kernel
+// attributes, device properties), done at cache creation so that no launch
+// performs it under stream capture.
+hipError_t prepare_fused(int hidden, int encoding, int loss);
+
+static_assert(sizeof(InferGroup) <= 4096 && sizeof(GradGroup) <= 4096 && sizeof(AdamGroup) <= 4096,
+              "grouped launch tables travel as kernel arguments");
 
 } // namespace nrc

@@ -17,15 +17,20 @@ namespace nrc {
 
 namespace {
 
-__global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
+__global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
+	// this block's cache: the last item whose first block is at or below blockIdx.x
+	uint32_t k = 0;
+	while (k + 1 < grp.n_items && blockIdx.x >= grp.item[k + 1].block_begin) ++k;
+	const AdamArgs &a = grp.item[k];
+	const uint32_t blk = blockIdx.x - a.block_begin;
 	const int t = threadIdx.x;
 	DeviceState *st = a.state;
 	const int p = a.parity;
 	const nrc_optimizer_state s0 = st->pub[p];
 	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
-	const bool loss_block = blockIdx.x == gridDim.x - 1;
+	const bool loss_block = blk == a.weight_count / 64u;
 	if (count == 0) { // nrc_optimize.comp:33-34: an empty batch changes nothing
-		if (blockIdx.x == 0 && t == 0) {
+		if (blk == 0 && t == 0) {
 			st->pub[p ^ 1] = s0;
 			st->overflow[p ^ 1] = 0;
 		}
@@ -35,7 +40,7 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
 
 	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 4*col4 .. +3
 	const int col4 = t & 15, slice = t >> 4;
-	const uint32_t w0 = blockIdx.x * 64u;
+	const uint32_t w0 = blk * 64u;
 	__shared__ float4 red[16][16];
 	float4 G4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	if (!skip) {
@@ -100,16 +105,16 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
 		a.w32[i] = e.weight;
 		const _Float16 h = (_Float16)e.weight;
 		a.w16[i] = h;
-		const uint32_t hid = (uint32_t)a.hidden * 4096u;
-		a.w16t[i < hid ? ((i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)) : i] = h;
 		frag_scatter(i, h, a.wf, a.wtf);
 		if (a.write_use) { // nrc_optimize.comp:50-53
 			const float u = a.p.use_ema ? e.ema_weight : e.weight;
 			a.use32[i] = u;
-			a.use16[i] = (_Float16)u;
+			const _Float16 uh = (_Float16)u;
+			a.use16[i] = uh;
+			frag_scatter(i, uh, a.usef, (_Float16 *)nullptr);
 		}
 	}
-	if (blockIdx.x == 0 && t == 0) {
+	if (blk == 0 && t == 0) {
 		nrc_optimizer_state s = s0;
 		if (skip) {
 			s.loss_scale *= 0.5f;
@@ -153,13 +158,6 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float *__restrict__ p
 	out[i] = s;
 }
 
-__global__ __launch_bounds__(256) void transpose_hidden_kernel(const _Float16 *__restrict__ w16,
-                                                                _Float16 *__restrict__ w16t, uint32_t n) {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	w16t[(i & ~4095u) | ((i & 63u) << 6) | ((i >> 6) & 63u)] = w16[i];
-}
-
 __global__ __launch_bounds__(256) void make_frags_kernel(const _Float16 *__restrict__ w16, _Float16 *__restrict__ wf,
                                                          _Float16 *__restrict__ wtf, uint32_t n) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,22 +185,28 @@ __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 
 } // namespace
 
-hipError_t launch_adam(const AdamArgs &a, hipStream_t stream) {
-	// one block per 64 weights plus one for the loss column
-	const dim3 grid(a.weight_count / 64 + 1), block(256);
-	adam_kernel<<<grid, block, 0, stream>>>(a);
+hipError_t launch_adam_group(AdamGroup &grp, hipStream_t stream) {
+	// per cache: one block per 64 weights plus one for the loss column
+	uint32_t blocks = 0;
+	for (uint32_t i = 0; i < grp.n_items; ++i) {
+		grp.item[i].block_begin = blocks;
+		blocks += grp.item[i].weight_count / 64 + 1;
+	}
+	if (blocks == 0) return hipSuccess;
+	adam_kernel<<<dim3(blocks), dim3(256), 0, stream>>>(grp);
 	return hipGetLastError();
+}
+
+hipError_t launch_adam(const AdamArgs &a, hipStream_t stream) {
+	AdamGroup grp{};
+	grp.n_items = 1;
+	grp.item[0] = a;
+	return launch_adam_group(grp, stream);
 }
 
 hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32_t pstride, uint32_t n, float *out,
                                   hipStream_t stream) {
 	reduce_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(partials, nparts, pstride, n, out);
-	return hipGetLastError();
-}
-
-hipError_t launch_transpose_hidden(int hidden, const _Float16 *w16, _Float16 *w16t, hipStream_t stream) {
-	const uint32_t n = (uint32_t)hidden * 4096u;
-	transpose_hidden_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(w16, w16t, n);
 	return hipGetLastError();
 }
 

@@ -62,6 +62,13 @@ class OptimizerState(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GroupItem(C.Structure):
+    """nrc_group_item: one cache's inputs and outputs in a grouped call."""
+    _fields_ = [("cache", C.c_void_p), ("d_queries", C.c_void_p), ("d_query_count", C.c_void_p),
+                ("max_queries", C.c_uint32), ("d_rgb", C.c_void_p), ("d_batches", C.c_void_p),
+                ("d_batch_counts", C.c_void_p), ("max_batch_records", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -97,9 +104,18 @@ def load_library():
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
     L.nrc_mlp_gradient_f16.argtypes = [U32, P, P, P, U32, P, P]
+    GI = C.POINTER(GroupItem)
+    L.nrc_group_infer.argtypes = [GI, U32, C.c_int, P]
+    L.nrc_group_train.argtypes = [GI, U32, U32, C.c_int, U32, P]
+    L.nrc_group_frame.argtypes = [GI, U32, C.c_int, P]
+    L.nrc_group_frame_graph.argtypes = [GI, U32, C.c_int, C.POINTER(P)]
+    L.nrc_graph_launch.argtypes = [P, P]
+    L.nrc_graph_destroy.argtypes = [P]
+    L.nrc_graph_destroy.restype = None
     for f in ("nrc_init_weights", "nrc_cache_create", "nrc_cache_reset", "nrc_cache_get_desc", "nrc_set_use_ema",
               "nrc_infer", "nrc_train", "nrc_frame", "nrc_get_state", "nrc_set_state", "nrc_encode",
-              "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16"):
+              "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16", "nrc_group_infer",
+              "nrc_group_train", "nrc_group_frame", "nrc_group_frame_graph", "nrc_graph_launch"):
         getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -264,6 +280,12 @@ class NRCState:
                 "ema_weight": entries[:, 3].copy(), "weights_f16": w16.view(np.float16).copy(),
                 "use_weights_f16": u16.view(np.float16).copy(), "state": st.as_dict()}
 
+    def get_optimizer_state(self) -> dict:
+        """The scalar optimizer state only (t, beta^t, alpha^t, loss scale, last loss); synchronous."""
+        st = OptimizerState()
+        _check(self._lib.nrc_get_state(self._h, None, C.byref(st), None, None))
+        return st.as_dict()
+
     def set_state(self, weight=None, m=None, v=None, ema_weight=None, state: dict | None = None) -> None:
         wc = self.weight_count
         ent = None
@@ -300,3 +322,63 @@ def mlp_forward_f16(weights_f16, inputs_f16, n: int, out, hidden_layers: int = 5
 def mlp_gradient_f16(weights_f16, inputs_f16, targets, n: int, dw, hidden_layers: int = 5, stream=None) -> None:
     _check(load_library().nrc_mlp_gradient_f16(int(hidden_layers), _ptr(weights_f16), _ptr(inputs_f16),
                                                _ptr(targets), int(n), _ptr(dw), _stream(stream)))
+
+
+# ---- many caches at once
+def _items(entries):
+    """Builds the nrc_group_item array from dicts with keys cache (NRCState), queries, query_count,
+    max_queries, rgb, batches, batch_counts, max_batch_records (all optional but cache)."""
+    arr = (GroupItem * max(1, len(entries)))()
+    keep = []
+    for i, e in enumerate(entries):
+        st = e["cache"]
+        q, rgb, b = e.get("queries"), e.get("rgb"), e.get("batches")
+        mq = e.get("max_queries", 0 if q is None else (rgb.shape[0] if rgb is not None else q.shape[0]))
+        mb = e.get("max_batch_records", 0 if b is None else b.shape[0] // TRAIN_BATCH_COUNT)
+        addr = lambda x: None if x is None else (x if isinstance(x, int) else x.data_ptr())  # noqa: E731
+        arr[i] = GroupItem(st._h.value, addr(q), addr(e.get("query_count")), int(mq), addr(rgb), addr(b),
+                           addr(e.get("batch_counts")), int(mb))
+        keep.append(e)
+    return arr, keep
+
+
+def group_infer(entries, path: int = PATH_FUSED_F16, stream=None) -> None:
+    arr, _ = _items(entries)
+    _check(load_library().nrc_group_infer(arr, len(entries), int(path), _stream(stream)))
+
+
+def group_train(entries, batch: int, path: int = PATH_FUSED_F16, write_use_weights: bool = False,
+                stream=None) -> None:
+    arr, _ = _items(entries)
+    flags = TRAIN_WRITE_USE_WEIGHTS if write_use_weights else 0
+    _check(load_library().nrc_group_train(arr, len(entries), int(batch), int(path), flags, _stream(stream)))
+
+
+def group_frame(entries, path: int = PATH_FUSED_F16, stream=None) -> None:
+    arr, _ = _items(entries)
+    _check(load_library().nrc_group_frame(arr, len(entries), int(path), _stream(stream)))
+
+
+class GroupGraph:
+    """A group frame captured once as a HIP graph (nrc_group_frame_graph) and replayed per frame."""
+
+    def __init__(self, entries, path: int = PATH_FUSED_F16):
+        self._lib = load_library()
+        arr, self._keep = _items(entries)
+        h = C.c_void_p()
+        _check(self._lib.nrc_group_frame_graph(arr, len(entries), int(path), C.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None) -> None:
+        _check(self._lib.nrc_graph_launch(self._h, _stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.nrc_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

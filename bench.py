@@ -37,6 +37,10 @@ TRAIN_FLOP_PER_SAMPLE = 115840    # forward 41,344 + dW 41,344 + dA 33,152
 QUERY_BYTES = 56 + 12             # 14-float record in, RGB out
 PEAK_F16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per fused_infer_kernel launch at this workload: rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0)
+# KiB per dispatch, scaled by a 256 MiB copy kernel measured in the same runs (bench.py --copy-calibration);
+# the commands and readings are in profiles/fused_infer_2073600q_hbm_traffic.md
+INFER_TRAFFIC_BYTES = (57007.0 * 2.0 + 24300.5) * 1024
 
 
 def log(*a):
@@ -169,26 +173,37 @@ def main() -> int:
     torch.cuda.synchronize()
 
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
     log(f"timing {K} frames ...")
+    fev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(K):
-        b = drec[(i % pool_frames) * BATCHES * BATCH:][:BATCHES * BATCH]
-        ev[i][0].record(stream)
+        fev[i][0].record(stream)
+        frame(i)
+        fev[i][1].record(stream)
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - t0
+    frame_ms = np.array([a.elapsed_time(b) for a, b in fev])
+    s = st.get_state()["state"]
+    log(f"loss {s['last_loss']:.4f} scale {s['loss_scale']} t {s['t']} skipped {s['skipped_steps']}")
+
+    # the inference kernel and the training steps alone (for the roofline and the split)
+    iev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    for i in range(K):
+        iev[i][0].record(stream)
         st.infer(dq, out, path=v.PATH_FUSED_F16, stream=stream)
-        ev[i][1].record(stream)
+        iev[i][1].record(stream)
+    tev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    for i in range(K):
+        b = drec[(i % pool_frames) * BATCHES * BATCH:][:BATCHES * BATCH]
+        tev[i][0].record(stream)
         for j in range(BATCHES):
             st.train(b[j * BATCH:(j + 1) * BATCH], path=v.PATH_FUSED_F16, write_use_weights=(j == BATCHES - 1),
                      stream=stream)
-        ev[i][2].record(stream)
+        tev[i][1].record(stream)
     torch.cuda.synchronize()
-    total_s = time.perf_counter() - t0
-    infer_ms = np.array([ev[i][0].elapsed_time(ev[i][1]) for i in range(K)])
-    train_ms = np.array([ev[i][1].elapsed_time(ev[i][2]) for i in range(K)])
-    frame_ms = infer_ms + train_ms
-    s = st.get_state()["state"]
-    log(f"loss {s['last_loss']:.4f} scale {s['loss_scale']} t {s['t']} skipped {s['skipped_steps']}")
+    infer_ms = np.array([a.elapsed_time(b) for a, b in iev])
+    train_ms = np.array([a.elapsed_time(b) for a, b in tev])
 
     # plain FP32 inference at the same size (the plain method)
     plain_ms = []
@@ -229,7 +244,7 @@ def main() -> int:
         "fused_vs_plain_infer": (QUERIES / infer_avg_s) / (QUERIES / (plain_ms / 1e3)),
         "roofline": {"bound": "mfma", "kernel": "fused_infer_kernel", "achieved": achieved,
                      "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_F16_TFLOPS,
-                     "traffic": None,
+                     "traffic": INFER_TRAFFIC_BYTES,
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
     }
     if args.group_frames > 0:

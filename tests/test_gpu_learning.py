@@ -7,7 +7,8 @@ FP32 path and the fused FP16-MFMA path:
     Adam lr 1e-2, beta2 0.99, batch 16384; the reference's
     test/mlp_learning_an_image/{gradient,optimize}.comp), also trained by the
     FP32 CPU port of the same algorithm (oracle/nrc_oracle.c built with
-    REAL=float), then evaluated by inference on 65,536 fresh uv;
+    REAL=float), then evaluated by inference on 65,536 fresh uv with the EMA
+    weights (SetUseEMAWeights), which average out the last steps' noise;
   * the radiance cache (SINCOS6 encoding, relative-L2 loss, dynamic loss scale
     starting at 128, 5 hidden layers).
 
@@ -49,7 +50,8 @@ def image_mse(pred, target):
 def test_image_mlp_learns_like_the_cpu_port_and_the_plain_path():
     img = wl.make_image(wl.config_seed(0, 1))
     kw = dict(hidden_layers=4, encoding=v.ENCODING_UV_SINCOS16, loss=v.LOSS_L2, init=v.INIT_XAVIER_UNIFORM,
-              seed=wl.config_seed(0), learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-8)
+              seed=wl.config_seed(0), learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-8,
+              use_ema_weights=True)
     plain = v.NRCState(v.NRCConfig(**kw))
     fused = v.NRCState(v.NRCConfig(**kw))
     w0 = plain.get_state()["weight"]
@@ -88,7 +90,7 @@ def test_image_mlp_learns_like_the_cpu_port_and_the_plain_path():
 
     mse_plain = gpu_mse(plain, v.PATH_PLAIN_F32)
     mse_fused = gpu_mse(fused, v.PATH_FUSED_F16)
-    mse_cpu = image_mse(cpu.infer(opt.w, 4, uv_eval, v.ENCODING_UV_SINCOS16, clamp0=True), target_eval)
+    mse_cpu = image_mse(cpu.infer(opt.ema, 4, uv_eval, v.ENCODING_UV_SINCOS16, clamp0=True), target_eval)
     print(f"image MSE: initial {mse0:.5f}  cpu {mse_cpu:.5f}  plain {mse_plain:.5f}  fused {mse_fused:.5f}")
     print("windowed loss cpu  ", np.round(windowed(lc), 5))
     print("windowed loss plain", np.round(windowed(lp), 5))

@@ -27,6 +27,9 @@ FLAGS = [
     "-fPIC",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs when they fit, so the fp16 pack reads them without
+    # v_accvgpr_read copies
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-command-line-argument",
 ]
 

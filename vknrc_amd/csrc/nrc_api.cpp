@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -234,6 +235,34 @@ nrc_status check_train(nrc_cache *c, const nrc_train_record *d_records, uint32_t
 	return NRC_OK;
 }
 
+
+// One training step on stream s (arguments already checked).
+nrc_status train_step(nrc_cache *c, const nrc_train_record *d_records, const uint32_t *d_count, uint32_t max_count,
+                      nrc_path path, uint32_t flags, hipStream_t s) {
+	nrc::TrainArgs a = train_args(c, d_records, d_count, max_count);
+	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
+	uint32_t nparts = 0;
+	hipError_t e;
+	if (path == NRC_PATH_PLAIN_F32) {
+		nrc_status st = ensure_plain_scratch(c);
+		if (st != NRC_OK) return st;
+		a.acts = c->acts;
+		a.deltas = c->deltas;
+		a.sample_loss = c->sample_loss;
+		a.ldn = c->ldn;
+		e = nrc::launch_plain_gradient(a, &nparts, s);
+	} else {
+		const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+		e = nrc::launch_fused_gradient(a, plan, s);
+		nparts = plan.blocks;
+	}
+	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
+	e = nrc::launch_adam(adam_args(c, d_count, max_count, nparts, flags), s);
+	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
+	c->parity ^= 1; // the optimizer wrote the other state buffer
+	return NRC_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -306,6 +335,7 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->state, sizeof(DeviceState));
 	if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(float) * (size_t)c->pstride * c->part_cap);
+
 	if (e != hipSuccess) {
 		free_cache(c);
 		return hip_fail(e, "nrc_cache_create");
@@ -373,43 +403,24 @@ nrc_status nrc_train(nrc_cache *c, const nrc_train_record *d_records, const uint
 	if (max_count == 0) return NRC_OK;
 	nrc_status st = check_train(c, d_records, max_count, path);
 	if (st != NRC_OK) return st;
-	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-	nrc::TrainArgs a = train_args(c, d_records, d_count, max_count);
-	const uint32_t n = max_count < a.clamp ? max_count : a.clamp;
-	uint32_t nparts = 0;
-	hipError_t e;
-	if (path == NRC_PATH_PLAIN_F32) {
-		st = ensure_plain_scratch(c);
-		if (st != NRC_OK) return st;
-		a.acts = c->acts;
-		a.deltas = c->deltas;
-		a.sample_loss = c->sample_loss;
-		a.ldn = c->ldn;
-		e = nrc::launch_plain_gradient(a, &nparts, s);
-	} else {
-		const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
-		e = nrc::launch_fused_gradient(a, plan, s);
-		nparts = plan.blocks;
-	}
-	if (e != hipSuccess) return hip_fail(e, "nrc_train gradient launch");
-	e = nrc::launch_adam(adam_args(c, d_count, max_count, nparts, flags), s);
-	if (e != hipSuccess) return hip_fail(e, "nrc_train optimizer launch");
-	c->parity ^= 1; // the optimizer wrote the other state buffer
-	return NRC_OK;
+	return train_step(c, d_records, d_count, max_count, path, flags, reinterpret_cast<hipStream_t>(stream));
 }
 
 nrc_status nrc_frame(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d_query_count, uint32_t max_queries,
                      float *d_rgb, const nrc_train_record *d_batches, const uint32_t *d_batch_counts,
                      uint32_t max_batch_records, nrc_path path, void *stream) {
-	nrc_status st = nrc_infer(c, d_queries, d_query_count, max_queries, d_rgb, path, stream);
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	if (max_queries && (!d_queries || !d_rgb)) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	nrc_status st = max_batch_records ? check_train(c, d_batches, max_batch_records, path) : NRC_OK;
 	if (st != NRC_OK) return st;
-	for (uint32_t b = 0; b < NRC_TRAIN_BATCH_COUNT; ++b) {
-		const uint32_t flags = (b == NRC_TRAIN_BATCH_COUNT - 1) ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u;
-		st = nrc_train(c, d_batches ? d_batches + (size_t)b * max_batch_records : nullptr,
-		               d_batch_counts ? d_batch_counts + b : nullptr, max_batch_records, path, flags, stream);
-		if (st != NRC_OK) return st;
-	}
-	return NRC_OK;
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	auto batch = [&](uint32_t b) { return d_batches + (size_t)b * max_batch_records; };
+	auto count = [&](uint32_t b) { return d_batch_counts ? d_batch_counts + b : nullptr; };
+	auto flags = [](uint32_t b) { return b == NRC_TRAIN_BATCH_COUNT - 1 ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u; };
+	st = nrc_infer(c, d_queries, d_query_count, max_queries, d_rgb, path, stream);
+	for (uint32_t b = 0; st == NRC_OK && max_batch_records && b < NRC_TRAIN_BATCH_COUNT; ++b)
+		st = train_step(c, batch(b), count(b), max_batch_records, path, flags(b), s);
+	return st;
 }
 
 nrc_status nrc_get_state(nrc_cache *c, nrc_optimizer_entry *entries, nrc_optimizer_state *state, uint16_t *w16,
@@ -570,6 +581,51 @@ std::vector<std::vector<uint32_t>> group_launches(const nrc_group_item *items, c
 	return out;
 }
 
+static nrc_status group_train_impl(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
+                                   uint32_t flags, hipStream_t s) {
+	nrc_status st = check_group(items, n_items, true);
+	if (st != NRC_OK) return st;
+	if (batch >= NRC_TRAIN_BATCH_COUNT) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch >= NRC_TRAIN_BATCH_COUNT");
+	std::vector<uint32_t> use;
+	for (uint32_t i = 0; i < n_items; ++i) {
+		const nrc_group_item &it = items[i];
+		if (it.max_batch_records == 0) continue;
+		st = check_train(it.cache, it.d_batches, it.max_batch_records, path);
+		if (st != NRC_OK) return st;
+		use.push_back(i);
+	}
+	auto records = [&](const nrc_group_item &it) { return it.d_batches + (size_t)batch * it.max_batch_records; };
+	auto counts = [&](const nrc_group_item &it) { return it.d_batch_counts ? it.d_batch_counts + batch : nullptr; };
+	if (path == NRC_PATH_PLAIN_F32) {
+		for (uint32_t i : use) {
+			const nrc_group_item &it = items[i];
+			st = train_step(it.cache, records(it), counts(it), it.max_batch_records, path, flags, s);
+			if (st != NRC_OK) return st;
+		}
+		return NRC_OK;
+	}
+	for (const auto &sel : group_launches(items, use, true)) {
+		nrc::GradGroup grad{};
+		nrc::AdamGroup adam{};
+		for (uint32_t i : sel) {
+			const nrc_group_item &it = items[i];
+			nrc_cache *c = it.cache;
+			const nrc::TrainArgs a = train_args(c, records(it), counts(it), it.max_batch_records);
+			const uint32_t n = it.max_batch_records < a.clamp ? it.max_batch_records : a.clamp;
+			const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+			grad.item[grad.n_items++] = nrc::grad_item(a, plan);
+			adam.item[adam.n_items++] = adam_args(c, counts(it), it.max_batch_records, plan.blocks, flags);
+		}
+		const nrc_cache_desc &d = items[sel[0]].cache->desc;
+		hipError_t e = nrc::launch_fused_gradient_group((int)d.hidden_layers, (int)d.encoding, (int)d.loss, grad, s);
+		if (e != hipSuccess) return hip_fail(e, "nrc_group_train gradient launch");
+		e = nrc::launch_adam_group(adam, s);
+		if (e != hipSuccess) return hip_fail(e, "nrc_group_train optimizer launch");
+		for (uint32_t i : sel) items[i].cache->parity ^= 1;
+	}
+	return NRC_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -610,55 +666,14 @@ nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_pa
 
 nrc_status nrc_group_train(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
                            uint32_t flags, void *stream) {
-	nrc_status st = check_group(items, n_items, true);
-	if (st != NRC_OK) return st;
-	if (batch >= NRC_TRAIN_BATCH_COUNT) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch >= NRC_TRAIN_BATCH_COUNT");
-	std::vector<uint32_t> use;
-	for (uint32_t i = 0; i < n_items; ++i) {
-		const nrc_group_item &it = items[i];
-		if (it.max_batch_records == 0) continue;
-		st = check_train(it.cache, it.d_batches, it.max_batch_records, path);
-		if (st != NRC_OK) return st;
-		use.push_back(i);
-	}
-	auto records = [&](const nrc_group_item &it) { return it.d_batches + (size_t)batch * it.max_batch_records; };
-	auto counts = [&](const nrc_group_item &it) { return it.d_batch_counts ? it.d_batch_counts + batch : nullptr; };
-	if (path == NRC_PATH_PLAIN_F32) {
-		for (uint32_t i : use) {
-			const nrc_group_item &it = items[i];
-			st = nrc_train(it.cache, records(it), counts(it), it.max_batch_records, path, flags, stream);
-			if (st != NRC_OK) return st;
-		}
-		return NRC_OK;
-	}
-	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-	for (const auto &sel : group_launches(items, use, true)) {
-		nrc::GradGroup grad{};
-		nrc::AdamGroup adam{};
-		for (uint32_t i : sel) {
-			const nrc_group_item &it = items[i];
-			nrc_cache *c = it.cache;
-			const nrc::TrainArgs a = train_args(c, records(it), counts(it), it.max_batch_records);
-			const uint32_t n = it.max_batch_records < a.clamp ? it.max_batch_records : a.clamp;
-			const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
-			grad.item[grad.n_items++] = nrc::grad_item(a, plan);
-			adam.item[adam.n_items++] = adam_args(c, counts(it), it.max_batch_records, plan.blocks, flags);
-		}
-		const nrc_cache_desc &d = items[sel[0]].cache->desc;
-		hipError_t e = nrc::launch_fused_gradient_group((int)d.hidden_layers, (int)d.encoding, (int)d.loss, grad, s);
-		if (e != hipSuccess) return hip_fail(e, "nrc_group_train gradient launch");
-		e = nrc::launch_adam_group(adam, s);
-		if (e != hipSuccess) return hip_fail(e, "nrc_group_train optimizer launch");
-		for (uint32_t i : sel) items[i].cache->parity ^= 1;
-	}
-	return NRC_OK;
+	return group_train_impl(items, n_items, batch, path, flags, reinterpret_cast<hipStream_t>(stream));
 }
 
 nrc_status nrc_group_frame(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
 	nrc_status st = nrc_group_infer(items, n_items, path, stream);
 	for (uint32_t b = 0; st == NRC_OK && b < NRC_TRAIN_BATCH_COUNT; ++b)
-		st = nrc_group_train(items, n_items, b, path, b == NRC_TRAIN_BATCH_COUNT - 1 ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u,
-		                     stream);
+		st = group_train_impl(items, n_items, b, path, b == NRC_TRAIN_BATCH_COUNT - 1 ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u,
+		                      reinterpret_cast<hipStream_t>(stream));
 	return st;
 }
 

@@ -164,8 +164,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 		// [0, count) instead of branching, so the load is unconditional (lanes past
 		// the count encode zeros regardless); a branch here makes the wait-count
 		// pass drain every outstanding load at the join.
-		const uint32_t last = count - 1;
+		// Prefetches past the wave's range re-read its last tile instead of the next
+		// wave's first one.
+		const uint32_t last = count - 1, tlast = lt1 - 1;
 		auto load_query = [&](uint32_t t, float *dst) __attribute__((always_inline)) {
+			t = t < tlast ? t : tlast;
 			uint32_t m = t * 64 + lane;
 			m = m < last ? m : last;
 			const float2 *qp = reinterpret_cast<const float2 *>(queries + (size_t)m * kQueryFloats);
@@ -317,6 +320,14 @@ template <int LD> __device__ __forceinline__ void store_bcol_ld(_Float16 *dst, c
 	for (int c = 0; c < 2; ++c)
 #pragma unroll
 		for (int j = 0; j < 8; ++j) dst[bfeat(c, g, j) * LD + col] = b[c][j];
+}
+
+// reads back this lane's 16 values of a B operand (record column col) written by store_bcol_ld
+template <int LD> __device__ __forceinline__ void load_bcol_ld(const _Float16 *src, half8_t *b, int g, int col) {
+#pragma unroll
+	for (int c = 0; c < 2; ++c)
+#pragma unroll
+		for (int j = 0; j < 8; ++j) b[c][j] = src[bfeat(c, g, j) * LD + col];
 }
 
 // delta = relu_bwd(acc) against the activations in B-operand layout, packed to fp16
@@ -493,11 +504,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		__syncthreads();
 
 		// ---- forward for this wave's 16 records; a_l kept as B operands and written feature-major
-		half8_t Bs[H + 1][2];
-#pragma unroll
-		for (int c = 0; c < 2; ++c)
-#pragma unroll
-			for (int j = 0; j < 8; ++j) Bs[0][c][j] = S.act[0][bfeat(c, g, j) * LD + col];
+		// (the activations are not kept in registers for the backward: its ReLU masks
+		// come back from the LDS copy, which leaves room for the weight fragments)
+		half8_t B[2];
+		load_bcol_ld<LD>(S.act[0], B, g, col);
 #pragma unroll
 		for (int l = 0; l < H; ++l) {
 			float4_t acc[4];
@@ -506,14 +516,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
-				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(Af[l][mt][c], Bs[l][c], acc[mt]);
+				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(Af[l][mt][c], B[c], acc[mt]);
 #pragma unroll
-			for (int c = 0; c < 2; ++c) Bs[l + 1][c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
-			store_bcol_ld<LD>(S.act[l + 1], Bs[l + 1], g, col);
+			for (int c = 0; c < 2; ++c) B[c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
+			store_bcol_ld<LD>(S.act[l + 1], B, g, col);
 		}
 		float4_t o = (float4_t)0.0f;
 #pragma unroll
-		for (int c = 0; c < 2; ++c) o = mfma(Ao[c], Bs[H][c], o);
+		for (int c = 0; c < 2; ++c) o = mfma(Ao[c], B[c], o);
 
 		// ---- loss gradient: lanes g == 0 hold the 3 outputs of record col
 		float go[3] = {0.0f, 0.0f, 0.0f};
@@ -560,7 +570,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			float4_t acc[4];
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
-			relu_bwd_pack(acc, Bs[H], Bd);
+			relu_bwd_pack(acc, B, Bd); // B still holds a_H
 			store_bcol_ld<LD>(S.dlt[H - 1], Bd, g, col);
 		}
 #pragma unroll
@@ -572,7 +582,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((l - 1) * 4 + mt) * 2 + c), Bd[c], acc[mt]);
-			relu_bwd_pack(acc, Bs[l], Bd);
+			half8_t al[2];
+			load_bcol_ld<LD>(S.act[l], al, g, col);
+			relu_bwd_pack(acc, al, Bd);
 			store_bcol_ld<LD>(S.dlt[l - 1], Bd, g, col);
 		}
 		// this wave's loss sum, in a fixed order
@@ -604,10 +616,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
 				for (int o2 = 0; o2 < 2; ++o2) {
+					if (undo != 1.0f) t[i][o2] *= undo; // uniform; exact when taken
 					if constexpr (MULTI)
-						accW[MULTI ? l : 0][i][o2] += t[i][o2] * undo;
+						accW[MULTI ? l : 0][i][o2] += t[i][o2];
 					else
-						store_tile(l, it0 + i, ot0 + o2, t[i][o2] * undo);
+						store_tile(l, it0 + i, ot0 + o2, t[i][o2]);
 				}
 		}
 		// dW_H^T: input tile `wave` against the output-gradient rows
@@ -620,7 +633,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				const half8_t b = *reinterpret_cast<const half8_t *>(S.go + r * LD + 32 * kc + 8 * g);
 				t = mfma(a, b, t);
 			}
-			accO += t * undo;
+			if (undo != 1.0f) t *= undo;
+			accO += t;
 		}
 	}
 
@@ -649,10 +663,21 @@ int infer_tiles() {
 	return t;
 }
 
+// Persistent inference workgroups per CU (VKNRC_INFER_WG_PER_CU overrides 2).
+uint32_t infer_wg_per_cu() {
+	static int v = 0;
+	if (v == 0) {
+		const char *e = getenv("VKNRC_INFER_WG_PER_CU");
+		v = e ? atoi(e) : 2;
+		if (v < 1) v = 1;
+	}
+	return (uint32_t)v;
+}
+
 template <int H, int ENC, bool F16IN> hipError_t infer_launch(const InferGroup &grp, hipStream_t s) {
-	// persistent: at most two workgroups per CU, each wave a contiguous tile range
+	// persistent: a few workgroups per CU, each wave a contiguous tile range
 	uint32_t blocks = (grp.total_tiles + 3) / 4;
-	const uint32_t cap = (uint32_t)device_cu_count() * 2;
+	const uint32_t cap = (uint32_t)device_cu_count() * infer_wg_per_cu();
 	if (blocks > cap) blocks = cap;
 	if (blocks == 0) blocks = 1;
 	if (infer_tiles() == 1)

@@ -2,7 +2,7 @@
 // optimizer scalar recurrences (nrc_optimize.comp:32-54, nrc_train_prepare.comp:16-29).
 //
 // Each block owns 64 consecutive weights; its 256 threads sum the per-workgroup
-// partial gradients as float4 columns in 16 interleaved slices (eight loads in
+// partial gradients as float4 columns in 16 interleaved slices (sixteen loads in
 // flight per thread) that are combined in a fixed order, so the reduced
 // gradient is bit-identical from run to run.  Weight counts are multiples of 64
 // and partial rows are 64-float aligned (the row stride holds the loss column and
@@ -46,19 +46,19 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	if (!skip) {
 		const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
 		const size_t ps4 = a.pstride / 4;
-		float4 acc[8];
+		float4 acc[16];
 #pragma unroll
-		for (int u = 0; u < 8; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		for (int u = 0; u < 16; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		uint32_t q = slice;
-		for (; q + 112 < a.nparts; q += 128) {
+		for (; q + 240 < a.nparts; q += 256) {
 #pragma unroll
-			for (int u = 0; u < 8; ++u) {
+			for (int u = 0; u < 16; ++u) {
 				const float4 v = src[(q + 16 * u) * ps4];
 				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
 			}
 		}
 #pragma unroll
-		for (int u = 0; u < 8; ++u) {
+		for (int u = 0; u < 16; ++u) {
 			if (q + 16 * u < a.nparts) {
 				const float4 v = src[(q + 16 * u) * ps4];
 				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 		}
 		G4 = acc[0];
 #pragma unroll
-		for (int u = 1; u < 8; ++u) {
+		for (int u = 1; u < 16; ++u) {
 			G4.x += acc[u].x; G4.y += acc[u].y; G4.z += acc[u].z; G4.w += acc[u].w;
 		}
 	}

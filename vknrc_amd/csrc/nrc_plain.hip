@@ -10,6 +10,10 @@ namespace nrc {
 
 namespace {
 
+// Output units between scheduling barriers in the FP32 forward: groups of 4
+// keep the H = 5 kernel within 256 VGPRs without spilling.
+constexpr int kPlainSchedGroup = 4;
+
 // x (64 input features) -> a_H in x, returns the 3 raw outputs in o.
 // a_{l+1} = relu(W_l a_l) (NN_nv.glsl:99-127), o = W_H a_H (NN_nv.glsl:129-158).
 template <int H> __device__ __forceinline__ void forward_f32(const float *sW, float *x, float *o) {
@@ -29,6 +33,8 @@ template <int H> __device__ __forceinline__ void forward_f32(const float *sW, fl
 				acc = fmaf(w.w, x[k + 3], acc);
 			}
 			y[m] = fmaxf(acc, 0.0f);
+			// bound how far the weight loads run ahead (and the registers they hold)
+			if (m % kPlainSchedGroup == kPlainSchedGroup - 1) __builtin_amdgcn_sched_barrier(0);
 		}
 #pragma unroll
 		for (int m = 0; m < kWidth; ++m) x[m] = y[m];

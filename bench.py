@@ -36,6 +36,7 @@ INFER_FLOP_PER_QUERY = 41344      # 2 * (5 * 64 * 64 + 3 * 64)
 TRAIN_FLOP_PER_SAMPLE = 115840    # forward 41,344 + dW 41,344 + dA 33,152
 QUERY_BYTES = 56 + 12             # 14-float record in, RGB out
 PEAK_F16_TFLOPS = 2500.0
+PEAK_F8_TFLOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per fused_infer_kernel launch at this workload: rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0)
 # KiB per dispatch, scaled by a 256 MiB copy kernel measured in the same runs (bench.py --copy-calibration);
@@ -205,6 +206,20 @@ def main() -> int:
     infer_ms = np.array([a.elapsed_time(b) for a, b in iev])
     train_ms = np.array([a.elapsed_time(b) for a, b in tev])
 
+    # FP8 (e4m3) inference of the same queries with the same use weights: time and error against FP16
+    st.infer(dq, out, path=v.PATH_FUSED_F16, stream=stream)
+    out16 = out.clone()
+    for _ in range(3):  # the activation-range history exists from the second call on
+        st.infer(dq, out, path=v.PATH_FUSED_F8, stream=stream)
+    f8ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    for i in range(K):
+        f8ev[i][0].record(stream)
+        st.infer(dq, out, path=v.PATH_FUSED_F8, stream=stream)
+        f8ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    f8_ms = np.array([a.elapsed_time(b) for a, b in f8ev])
+    f8_err = float(torch.linalg.vector_norm((out - out16).double()) / torch.linalg.vector_norm(out16.double()))
+
     # plain FP32 inference at the same size (the plain method)
     plain_ms = []
     for _ in range(args.plain_reps + 1):
@@ -247,6 +262,11 @@ def main() -> int:
                      "traffic": INFER_TRAFFIC_BYTES,
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
     }
+    f8_s = float(f8_ms.mean()) / 1e3
+    f8_tflops = INFER_FLOP_PER_QUERY * QUERIES / f8_s / 1e12
+    res["fp8"] = {"infer_queries_per_s": QUERIES / f8_s, "infer_ms_avg": f8_s * 1e3,
+                  "speedup_vs_f16": infer_avg_s / f8_s, "rel_err_vs_f16": f8_err,
+                  "achieved_TFLOPs": f8_tflops, "peak_TFLOPs": PEAK_F8_TFLOPS, "frac": f8_tflops / PEAK_F8_TFLOPS}
     if args.group_frames > 0:
         log("16-cache workload ...")
         res["group16"] = group_workload(torch, v, wl, q, args.group_frames)

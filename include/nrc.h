@@ -42,8 +42,12 @@ typedef enum nrc_status {
 
 /* Kernel form.  PLAIN_F32 is the plain method: FP32 features, weights and
  * accumulation.  FUSED_F16 runs the layer GEMMs on FP16 MFMA with FP32
- * accumulation, FP16 activations and FP32 master weights. */
-typedef enum nrc_path { NRC_PATH_PLAIN_F32 = 0, NRC_PATH_FUSED_F16 = 1 } nrc_path;
+ * accumulation, FP16 activations and FP32 master weights.  FUSED_F8 runs the
+ * inference forward on FP8 (OCP e4m3) MFMA with per-layer power-of-two scales
+ * chosen from the largest activations of the cache's last 16 FP8 inferences;
+ * its training steps are those of FUSED_F16 (FP16 backward, FP32 master
+ * weights).  No counterpart in the reference, which runs FP16 throughout. */
+typedef enum nrc_path { NRC_PATH_PLAIN_F32 = 0, NRC_PATH_FUSED_F16 = 1, NRC_PATH_FUSED_F8 = 2 } nrc_path;
 
 /* Input encoding (NRCRecord.glsl:78-95).  TRI12 is the reference layout.
  * SINCOS6 replaces each position axis's 12 triangle waves by sin(2^k pi p),
@@ -177,6 +181,15 @@ nrc_status nrc_get_state(nrc_cache *cache, nrc_optimizer_entry *host_entries, nr
                          uint16_t *host_weights_f16, uint16_t *host_use_weights_f16);
 nrc_status nrc_set_state(nrc_cache *cache, const nrc_optimizer_entry *host_entries,
                          const nrc_optimizer_state *host_state);
+
+/* The FP8 scales the cache's last FP8 inference ran with (synchronous):
+ * layer_scales[l] = C_l for l = 0..hidden_layers (the activations a_l enter
+ * layer l as C_l a_l in e4m3, layer l's weights as (C_{l+1} / C_l) W_l), and
+ * out_scale = 1 / (c_H C_H) with c_H the output layer's weight scale;
+ * frames = FP8 inferences recorded in the 16-frame activation history.
+ * layer_scales needs hidden_layers + 1 entries; any pointer may be NULL.
+ * No counterpart in the reference, which has no FP8 path. */
+nrc_status nrc_get_fp8_scales(nrc_cache *cache, float *layer_scales, float *out_scale, uint32_t *frames);
 
 /* NRCInputEncode (NRCRecord.glsl:78-95) alone: d_features is [n][64] FP32
  * (what the plain path feeds its first layer). */

@@ -22,6 +22,8 @@ struct nrc_cache {
 	_Float16 *w16 = nullptr, *use16 = nullptr;
 	_Float16 *wf = nullptr, *wtf = nullptr; // fragment-major copies for the fused gradient
 	_Float16 *usef = nullptr;               // fragment-major use-weights for the fused inference
+	uint8_t *w8 = nullptr;                  // fragment-major e4m3 use-weights (FP8 inference)
+	nrc::Fp8State *f8 = nullptr;            // FP8 scales and activation history
 	float *w32 = nullptr, *use32 = nullptr;
 	DeviceState *state = nullptr;
 	float *partials = nullptr;
@@ -135,6 +137,9 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 	nrc_status s = upload_entries(c, e.data());
 	if (s != NRC_OK) return s;
 	NRC_HIP(nrc::launch_init_state(c->state, c->desc.loss_scale, nullptr), "init state");
+	// FP8: no scales and no activation history yet; the padding of the fp8 copy stays zero
+	NRC_HIP(hipMemset(c->f8, 0, sizeof(nrc::Fp8State)), "init fp8 state");
+	NRC_HIP(hipMemset(c->w8, 0, (size_t)nrc::fwd_frag_halves((int)c->desc.hidden_layers)), "init fp8 weights");
 	c->parity = 0;
 	NRC_HIP(hipDeviceSynchronize(), "reset");
 	c->desc.seed = seed;
@@ -144,7 +149,7 @@ nrc_status reset_cache(nrc_cache *c, uint64_t seed) {
 void free_cache(nrc_cache *c) {
 	if (!c) return;
 	(void)hipSetDevice(c->device);
-	void *ptrs[] = {c->entries, c->w16, c->wf, c->wtf, c->usef, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
+	void *ptrs[] = {c->entries, c->w16, c->wf, c->wtf, c->usef, c->w8, c->f8, c->use16, c->w32, c->use32, c->state, c->partials, c->acts, c->deltas, c->sample_loss};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	delete c;
@@ -226,7 +231,8 @@ nrc::AdamArgs adam_args(nrc_cache *c, const uint32_t *d_count, uint32_t max_coun
 nrc_status check_train(nrc_cache *c, const nrc_train_record *d_records, uint32_t max_count, nrc_path path) {
 	if (!d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL records");
 	if (max_count > NRC_MAX_TRAIN_BATCH_SIZE) return fail(NRC_ERROR_CAPACITY, "max_count > NRC_MAX_TRAIN_BATCH_SIZE");
-	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16 && path != NRC_PATH_FUSED_F8)
+		return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	const uint32_t clamp = c->desc.train_batch_size;
 	const uint32_t n = max_count < clamp ? max_count : clamp;
 	const uint32_t need = path == NRC_PATH_PLAIN_F32 ? (n + nrc::kPlainChunk - 1) / nrc::kPlainChunk
@@ -330,6 +336,8 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	if (e == hipSuccess) e = hipMalloc(&c->wf, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
 	if (e == hipSuccess) e = hipMalloc(&c->wtf, 2 * (size_t)nrc::bwd_frag_halves((int)desc->hidden_layers));
 	if (e == hipSuccess) e = hipMalloc(&c->usef, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
+	if (e == hipSuccess) e = hipMalloc(&c->w8, (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
+	if (e == hipSuccess) e = hipMalloc(&c->f8, sizeof(nrc::Fp8State));
 	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
 	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
@@ -383,17 +391,37 @@ nrc_status nrc_infer(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d
 	hipError_t e;
 	switch (path) {
 	case NRC_PATH_PLAIN_F32: e = nrc::launch_plain_infer(H, enc, q, d_count, max_count, c->use32, d_rgb, s); break;
-	case NRC_PATH_FUSED_F16: {
+	case NRC_PATH_FUSED_F16:
+	case NRC_PATH_FUSED_F8: {
 		nrc::InferGroup grp{};
 		grp.n_items = 1;
 		grp.clamp0 = 1;
-		grp.item[0] = nrc::InferItem{q, d_count, d_rgb, c->usef, max_count, 0};
-		e = nrc::launch_fused_infer_group(H, enc, grp, s);
+		grp.item[0] = nrc::InferItem{q, d_count, d_rgb, c->usef, c->w8, c->f8, max_count, 0};
+		if (path == NRC_PATH_FUSED_F16) {
+			e = nrc::launch_fused_infer_group(H, enc, grp, s);
+		} else {
+			const float *u32[1] = {c->use32};
+			uint8_t *w8[1] = {c->w8};
+			e = nrc::launch_fp8_infer_group(H, enc, grp, u32, w8, s);
+		}
 		break;
 	}
 	default: return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_infer launch");
+	return NRC_OK;
+}
+
+nrc_status nrc_get_fp8_scales(nrc_cache *c, float *layer_scales, float *out_scale, uint32_t *frames) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	nrc::Fp8State st;
+	NRC_HIP(hipSetDevice(c->device), "hipSetDevice");
+	NRC_HIP(hipDeviceSynchronize(), "nrc_get_fp8_scales");
+	NRC_HIP(hipMemcpy(&st, c->f8, sizeof(st), hipMemcpyDeviceToHost), "download fp8 state");
+	if (layer_scales)
+		for (uint32_t l = 0; l <= c->desc.hidden_layers; ++l) layer_scales[l] = st.scale[l];
+	if (out_scale) *out_scale = st.out_scale;
+	if (frames) *frames = st.frames;
 	return NRC_OK;
 }
 
@@ -633,7 +661,8 @@ extern "C" {
 nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
 	nrc_status st = check_group(items, n_items, false);
 	if (st != NRC_OK) return st;
-	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16) return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16 && path != NRC_PATH_FUSED_F8)
+		return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	std::vector<uint32_t> use;
 	for (uint32_t i = 0; i < n_items; ++i) {
 		if (items[i].max_queries == 0) continue;
@@ -652,13 +681,20 @@ nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_pa
 	for (const auto &sel : group_launches(items, use, false)) {
 		nrc::InferGroup grp{};
 		grp.clamp0 = 1;
+		const float *u32[nrc::kGroupMax];
+		uint8_t *w8[nrc::kGroupMax];
 		for (uint32_t i : sel) {
 			const nrc_group_item &it = items[i];
-			grp.item[grp.n_items++] = nrc::InferItem{reinterpret_cast<const float *>(it.d_queries), it.d_query_count,
-			                                         it.d_rgb, it.cache->usef, it.max_queries, 0};
+			u32[grp.n_items] = it.cache->use32;
+			w8[grp.n_items] = it.cache->w8;
+			grp.item[grp.n_items++] =
+			    nrc::InferItem{reinterpret_cast<const float *>(it.d_queries), it.d_query_count, it.d_rgb,
+			                   it.cache->usef, it.cache->w8, it.cache->f8, it.max_queries, 0};
 		}
 		const nrc_cache_desc &d = items[sel[0]].cache->desc;
-		hipError_t e = nrc::launch_fused_infer_group((int)d.hidden_layers, (int)d.encoding, grp, s);
+		hipError_t e = path == NRC_PATH_FUSED_F8
+		                   ? nrc::launch_fp8_infer_group((int)d.hidden_layers, (int)d.encoding, grp, u32, w8, s)
+		                   : nrc::launch_fused_infer_group((int)d.hidden_layers, (int)d.encoding, grp, s);
 		if (e != hipSuccess) return hip_fail(e, "nrc_group_infer launch");
 	}
 	return NRC_OK;

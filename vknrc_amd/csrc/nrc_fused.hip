@@ -1,116 +1,20 @@
-// nrc_fused.hip -- fully fused 64-wide MLP on gfx950 FP16 MFMA (FP32 accumulation).
-//
-// MFMA used: v_mfma_f32_16x16x32_f16.  Per instruction, lane l = 16*g + r holds
-//   A: row r,    8 halves of K-group g      B: column r, 8 halves of K-group g
-//   D: column r, rows 4g .. 4g+3 (FP32)
-// A and B pair their halves by (g, j), so any bijection between (g, j) and the
-// 32 K indices of an instruction gives the same product as long as A and B use
-// the same one.  The kernels use
-//   k(c, g, j) = 32c + 4g + j        (j < 4)
-//              = 32c + 16 + 4g + j-4 (j >= 4)        for K-chunk c in {0, 1}
-// because that is exactly where the D tiles of the previous layer leave the
-// features: D of M-tile 2c holds rows 32c + 4g + i, D of M-tile 2c+1 holds rows
-// 32c + 16 + 4g + i.  Layer l's output therefore becomes layer l+1's B operand
-// in the same lanes, with a ReLU and an FP16 pack and no data movement.
+// nrc_fused.hip -- fully fused 64-wide MLP on gfx950 FP16 MFMA (FP32 accumulation):
+// the grouped inference and the grouped gradient kernels.  The MFMA operand
+// layouts and the K permutation that chains the layers are described in
+// nrc_mfma.hpp.
 //
 // Weight layout in HBM (unchanged from the reference, NN_nv.glsl:55-57): the
 // fp16 weights of layer l, row-major [out][in], at l*4096; the 3x64 output
 // layer at H*4096 (padded to 16 rows with zeros inside the MFMA, NN_nv.glsl:77-82).
+// The kernels read the fragment-major copies of nrc_device.hpp.
 #include <cstdlib>
 #include <type_traits>
 
-#include "nrc_internal.hpp"
+#include "nrc_mfma.hpp"
 
 namespace nrc {
 
 namespace {
-
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float4_t mfma(half8_t a, half8_t b, float4_t c) {
-	return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-typedef float float2_t __attribute__((ext_vector_type(2)));
-
-// Two FP32 values -> packed FP16 (round to nearest even), then ReLU on the pair.
-// max(cvt(x), 0) equals cvt(max(x, 0)) up to the sign of a zero.
-__device__ __forceinline__ half2_t cvt_relu2(float a, float b) {
-	const float2_t f = {a, b};
-	return __builtin_elementwise_max(__builtin_convertvector(f, half2_t), (half2_t)0);
-}
-
-__device__ __forceinline__ half8_t pack_relu(float4_t lo, float4_t hi) {
-	const half2_t a = cvt_relu2(lo[0], lo[1]), b = cvt_relu2(lo[2], lo[3]);
-	const half2_t c = cvt_relu2(hi[0], hi[1]), d = cvt_relu2(hi[2], hi[3]);
-	half8_t h;
-	h[0] = a[0]; h[1] = a[1]; h[2] = b[0]; h[3] = b[1];
-	h[4] = c[0]; h[5] = c[1]; h[6] = d[0]; h[7] = d[1];
-	return h;
-}
-
-// Orders LDS traffic between lanes of one wavefront (LDS operations of a wave
-// execute in order; this keeps the compiler from moving them across).  The
-// fences are restricted to LDS so they wait on LDS counters only and leave the
-// prefetched global loads in flight.
-__device__ __forceinline__ void wave_lds_sync() {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-}
-
-// --------------------------------------------------------------- encoders
-// Emits the 16 feature quads (features 4p..4p+3) of one record; Emit(p, f0..f3).
-template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(const float *rec, Emit &&emit) {
-	if constexpr (ENC == NRC_ENCODING_UV_SINCOS16) {
-#pragma unroll
-		for (int a = 0; a < 2; ++a) {
-#pragma unroll
-			for (int k0 = 0; k0 < 16; k0 += 4) {
-				float s[4], c[4];
-				float sv, cv;
-				sincospi_poly(rec[a] * (float)(1 << k0), &sv, &cv);
-				s[0] = sv;
-				c[0] = cv;
-#pragma unroll
-				for (int k = 1; k < 4; ++k) {
-					const float ns = 2.0f * sv * cv, nc = 1.0f - 2.0f * sv * sv;
-					sv = ns;
-					cv = nc;
-					s[k] = sv;
-					c[k] = cv;
-				}
-				emit(8 * a + k0 / 4, s[0], s[1], s[2], s[3]);
-				emit(8 * a + 4 + k0 / 4, c[0], c[1], c[2], c[3]);
-			}
-		}
-	} else {
-#pragma unroll
-		for (int a = 0; a < 3; ++a) {
-			float t[12];
-			axis12<ENC>(rec[a], t);
-			emit(3 * a + 0, t[0], t[1], t[2], t[3]);
-			emit(3 * a + 1, t[4], t[5], t[6], t[7]);
-			emit(3 * a + 2, t[8], t[9], t[10], t[11]);
-		}
-		float b[4];
-		oneblob4(rec[3], b);
-		emit(9, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[4], b);
-		emit(10, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[5], b);
-		emit(11, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[6], b);
-		emit(12, b[0], b[1], b[2], b[3]);
-		oneblob4(1.0f - expf(-rec[7]), b);
-		emit(13, b[0], b[1], b[2], b[3]);
-		emit(14, rec[8], rec[9], rec[10], rec[11]);
-		emit(15, rec[12], rec[13], 1.0f, 1.0f);
-	}
-}
 
 // ================================================================ inference
 //
@@ -282,8 +186,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 	}
 }
 
-// feature index of element j of a B operand held by lane group g for K-chunk c
-__device__ __forceinline__ int bfeat(int c, int g, int j) { return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4); }
+
+// The same inference, software-pipelined: a wave encodes tile t+1 into one LDS
+// slot while the layers of tile t run from the other, and nothing between two
+// slot switches branches (records past the count encode the last record, and
+// the outputs go out through buffer stores whose range check drops the lanes
+// that hold no output and the records past the count), so the compiler can
+// interleave the encoding's VALU work with the MFMA chains of the 4 N-tiles.
+typedef unsigned int uint3_t __attribute__((ext_vector_type(3)));
+
+template <int H, int ENC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void fused_infer_pipe_kernel(const InferGroup grp) {
+	// two separate arrays, so the compiler knows the encoding's stores into one
+	// slot never alias the layers' loads from the other
+	__shared__ __attribute__((aligned(16))) half8_t sX0[4][8 * 64];
+	__shared__ __attribute__((aligned(16))) half8_t sX1[4][8 * 64];
+
+	// the wave index and everything derived from it are wave-uniform: in SGPRs
+	const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int g = lane >> 4, r = lane & 15;
+	const uint32_t nW = gridDim.x * 4, gw = blockIdx.x * 4 + wave;
+	const uint32_t t0 = (uint32_t)((uint64_t)grp.total_tiles * gw / nW);
+	const uint32_t t1 = (uint32_t)((uint64_t)grp.total_tiles * (gw + 1) / nW);
+	half8_t *slot0 = sX0[wave], *slot1 = sX1[wave];
+	const int et = lane >> 4, es = lane & 15; // this lane's record: N-tile et, column es
+	const float lo = grp.clamp0 ? 0.0f : -__builtin_huge_valf();
+
+	for (uint32_t i = 0; i < grp.n_items; ++i) {
+		const InferItem &it = grp.item[i];
+		const uint32_t b0 = it.tile_begin, b1 = b0 + (it.max_count + 63) / 64;
+		if (b1 <= t0) continue;
+		if (b0 >= t1) break;
+		const uint32_t count = __builtin_amdgcn_readfirstlane(effective_count(it.d_count, it.max_count, it.max_count));
+		const uint32_t lt0 = (t0 > b0 ? t0 : b0) - b0;
+		uint32_t lt1 = (t1 < b1 ? t1 : b1) - b0;
+		const uint32_t ntiles = (count + 63) / 64;
+		lt1 = lt1 < ntiles ? lt1 : ntiles;
+		if (lt0 >= lt1) continue;
+
+		const half8_t *wv = reinterpret_cast<const half8_t *>(it.wf);
+		half8_t A[H][4][2], Ao[2];
+#pragma unroll
+		for (int l = 0; l < H; ++l)
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+				for (int c = 0; c < 2; ++c) A[l][mt][c] = wv[((l * 4 + mt) * 2 + c) * 64 + lane];
+#pragma unroll
+		for (int c = 0; c < 2; ++c) Ao[c] = wv[(H * 8 + c) * 64 + lane]; // rows 3..15 are zero
+
+		const float *queries = it.queries;
+		float *out = it.out;
+		const uint32_t last = count - 1, tlast = lt1 - 1;
+		auto load_query = [&](uint32_t t, float *dst) __attribute__((always_inline)) {
+			t = t < tlast ? t : tlast;
+			uint32_t m = t * 64 + lane;
+			m = m < last ? m : last;
+			const float2 *qp = reinterpret_cast<const float2 *>(queries + (size_t)m * kQueryFloats);
+#pragma unroll
+			for (int k = 0; k < 7; ++k) {
+				const float2 v = qp[k];
+				dst[2 * k] = v.x;
+				dst[2 * k + 1] = v.y;
+			}
+		};
+		auto emitter = [&](half8_t *slot) __attribute__((always_inline)) {
+			half4_t *s4 = reinterpret_cast<half4_t *>(slot);
+			return [=](int p, float f0, float f1, float f2, float f3) __attribute__((always_inline)) {
+				const int c = p >> 3, gg = p & 3, hi = (p >> 2) & 1;
+				half4_t h;
+				h[0] = (_Float16)f0; h[1] = (_Float16)f1; h[2] = (_Float16)f2; h[3] = (_Float16)f3;
+				s4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = h;
+			};
+		};
+		constexpr int NP = encode_pieces<ENC>();
+		// piece k of the next tile's encoding goes after the MFMAs of step s = k * steps / NP
+		auto piece_at = [&](auto step_c, const float *rec, half8_t *slot) __attribute__((always_inline)) {
+			constexpr int step = decltype(step_c)::value;
+			constexpr int steps = 4 * (H + 1);
+			static_for<NP>([&](auto k_c) __attribute__((always_inline)) {
+				constexpr int k = decltype(k_c)::value;
+				if constexpr (k * steps / NP == step) encode_piece<ENC, k>(rec, emitter(slot));
+			});
+		};
+		// the layers of the tile staged in `cur`, with the encoding of the next
+		// tile (records `rec`) into `nxt` spread between them
+		auto run_tile = [&](const half8_t *cur, uint32_t tile, const float *rec, half8_t *nxt) __attribute__((always_inline)) {
+			const uint32_t left = count - tile * 64;
+			const uint32_t nrec = left < 64u ? left : 64u;
+			const __amdgpu_buffer_rsrc_t rs =
+			    __builtin_amdgcn_make_buffer_rsrc(out + (size_t)tile * 64 * kOut, (short)0, (int)(nrec * kOut * 4), 0x00020000);
+			static_for<4>([&](auto nt_c) __attribute__((always_inline)) {
+				constexpr int nt = decltype(nt_c)::value;
+				half8_t B[2];
+#pragma unroll
+				for (int c = 0; c < 2; ++c) B[c] = cur[(nt * 2 + c) * 64 + lane];
+				static_for<H>([&](auto l_c) __attribute__((always_inline)) {
+					constexpr int l = decltype(l_c)::value;
+					float4_t acc[4];
+#pragma unroll
+					for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
+#pragma unroll
+					for (int c = 0; c < 2; ++c)
+#pragma unroll
+						for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(A[l][mt][c], B[c], acc[mt]);
+					piece_at(std::integral_constant<int, nt * (H + 1) + l>{}, rec, nxt);
+#pragma unroll
+					for (int c = 0; c < 2; ++c) B[c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
+				});
+				float4_t o = (float4_t)0.0f;
+#pragma unroll
+				for (int c = 0; c < 2; ++c) o = mfma(Ao[c], B[c], o);
+				piece_at(std::integral_constant<int, nt * (H + 1) + H>{}, rec, nxt);
+				uint3_t v;
+				v[0] = __float_as_uint(fmaxf(o[0], lo));
+				v[1] = __float_as_uint(fmaxf(o[1], lo));
+				v[2] = __float_as_uint(fmaxf(o[2], lo));
+				const uint32_t off = g == 0 ? (uint32_t)((nt * 16 + r) * kOut * 4) : 0x7ffffff0u;
+				__builtin_amdgcn_raw_buffer_store_b96(v, rs, off, 0, 0);
+			});
+		};
+
+		// records: ra holds tile t (or t+2), rb tile t+1 (or t+3)
+		float ra[kQueryFloats], rb[kQueryFloats];
+		load_query(lt0, ra);
+		load_query(lt0 + 1, rb);
+		encode_quads<ENC>(ra, emitter(slot0));
+		wave_lds_sync();
+		for (uint32_t t = lt0; t < lt1; t += 2) {
+			load_query(t + 2, ra);
+			run_tile(slot0, t, rb, slot1); // encodes tile t+1 (a repeat of the last tile past the range)
+			wave_lds_sync();
+			if (t + 1 >= lt1) break;
+			load_query(t + 3, rb);
+			run_tile(slot1, t + 1, ra, slot0); // encodes tile t+2
+			wave_lds_sync();
+		}
+	}
+}
 
 // ================================================================ gradient
 //
@@ -663,15 +703,15 @@ int infer_tiles() {
 	return t;
 }
 
-// Persistent inference workgroups per CU (VKNRC_INFER_WG_PER_CU overrides 2).
-uint32_t infer_wg_per_cu() {
-	static int v = 0;
-	if (v == 0) {
-		const char *e = getenv("VKNRC_INFER_WG_PER_CU");
-		v = e ? atoi(e) : 2;
-		if (v < 1) v = 1;
+
+// The software-pipelined inference kernel (VKNRC_INFER_PIPE=0 selects the other).
+bool infer_pipelined() {
+	static int p = -1;
+	if (p < 0) {
+		const char *e = getenv("VKNRC_INFER_PIPE");
+		p = (e && atoi(e) == 0) ? 0 : 1;
 	}
-	return (uint32_t)v;
+	return p == 1;
 }
 
 template <int H, int ENC, bool F16IN> hipError_t infer_launch(const InferGroup &grp, hipStream_t s) {
@@ -680,7 +720,9 @@ template <int H, int ENC, bool F16IN> hipError_t infer_launch(const InferGroup &
 	const uint32_t cap = (uint32_t)device_cu_count() * infer_wg_per_cu();
 	if (blocks > cap) blocks = cap;
 	if (blocks == 0) blocks = 1;
-	if (infer_tiles() == 1)
+	if (!F16IN && infer_pipelined())
+		fused_infer_pipe_kernel<H, ENC><<<dim3(blocks), dim3(256), 0, s>>>(grp);
+	else if (infer_tiles() == 1)
 		fused_infer_kernel<H, ENC, F16IN, 1><<<dim3(blocks), dim3(256), 0, s>>>(grp);
 	else
 		fused_infer_kernel<H, ENC, F16IN, 2><<<dim3(blocks), dim3(256), 0, s>>>(grp);
@@ -742,6 +784,16 @@ template <int H> hipError_t infer_h(int enc, const InferGroup &grp, hipStream_t 
 }
 
 } // namespace
+
+uint32_t infer_wg_per_cu() {
+	static int v = 0;
+	if (v == 0) {
+		const char *e = getenv("VKNRC_INFER_WG_PER_CU");
+		v = e ? atoi(e) : 2;
+		if (v < 1) v = 1;
+	}
+	return (uint32_t)v;
+}
 
 int device_cu_count() {
 	static int cus = 0;

@@ -19,15 +19,43 @@ constexpr uint32_t kFusedMaxChunks = 16;
 // Most caches one grouped launch takes (the tables travel as kernel arguments).
 constexpr uint32_t kGroupMax = 16;
 
+// FP8 inference state of one cache (device-resident).  Activations a_l are
+// carried scaled by C_l (C_0 = 1, C_{l+1} = C_l c_l), the layer-l weights by c_l
+// (powers of two); the output is unscaled by out_scale = 1 / (c_H C_H).
+constexpr int kFp8History = 16;  // frames of activation maxima
+constexpr int kFp8Layers = 6;    // a_0 .. a_H for H <= 5
+constexpr int kFp8MaxBlocks = 1024;
+struct Fp8State {
+	float hist[kFp8History][kFp8Layers];     // largest |a_l| (unscaled) of past frames
+	float block_amax[kFp8MaxBlocks][kFp8Layers]; // this frame: per inference block, scaled
+	float scale[kFp8Layers];                  // C_l of the weights now in the fp8 copy
+	float wscale[kFp8Layers];                 // c_l = C_{l+1} / C_l (c_H for the output layer)
+	float out_scale;
+	uint32_t frames; // frames recorded into hist
+};
+
 // One cache's share of a grouped fused inference.
 struct InferItem {
 	const float *queries;    // [max_count][14]
 	const uint32_t *d_count; // device count or null (= max_count)
 	float *out;              // [max_count][3]
-	const _Float16 *wf;      // fragment-major use-weights
+	const _Float16 *wf;      // fragment-major use-weights (FP16 path)
+	const uint8_t *w8;       // fragment-major fp8 use-weights (FP8 path)
+	Fp8State *f8;            // FP8 path: scales in, activation maxima out
 	uint32_t max_count;
 	uint32_t tile_begin; // first 64-record tile of this cache in the group (set by the launcher)
 };
+// One cache's FP8 weight preparation.
+struct Fp8PrepItem {
+	Fp8State *f8;
+	const float *use32; // FP32 use-weights, row-major
+	uint8_t *w8;        // fragment-major e4m3 copy of the scaled use-weights
+};
+struct Fp8PrepGroup {
+	Fp8PrepItem item[kGroupMax];
+	uint32_t n_items;
+};
+
 struct InferGroup {
 	InferItem item[kGroupMax];
 	const _Float16 *fin; // pre-encoded fp16 inputs [n][64] (single-item test entry point only)
@@ -122,6 +150,11 @@ hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Flo
 // nrc_fused.hip
 // All items share (hidden, encoding); fills tile_begin and total_tiles.
 hipError_t launch_fused_infer_group(int hidden, int encoding, InferGroup &grp, hipStream_t stream);
+// FP8 path: per item, folds the last inference's activation maxima into the
+// history, picks the layer scales and requantizes the fp8 use-weights from
+// use32; then the grouped FP8 inference.
+hipError_t launch_fp8_infer_group(int hidden, int encoding, InferGroup &grp, const float *const *use32,
+                                  uint8_t *const *w8, hipStream_t stream);
 // Forward over n pre-encoded fp16 input rows with fragment-major weights wf (no clamp).
 hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *wf, float *out,
                                     hipStream_t stream);
@@ -138,6 +171,8 @@ hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const f
                                      uint32_t *nparts, hipStream_t stream);
 
 int device_cu_count();
+// Persistent inference workgroups per CU (VKNRC_INFER_WG_PER_CU overrides 2).
+uint32_t infer_wg_per_cu();
 // One-time launch set-up of the fused kernels a cache of this kind uses (kernel
 // attributes, device properties), done at cache creation so that no launch
 // performs it under stream capture.

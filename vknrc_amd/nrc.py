@@ -29,7 +29,7 @@ TRAIN_BATCH_COUNT = 4
 DEFAULT_TRAIN_BATCH_SIZE = 16384
 MAX_TRAIN_BATCH_SIZE = 65536
 
-PATH_PLAIN_F32, PATH_FUSED_F16 = 0, 1
+PATH_PLAIN_F32, PATH_FUSED_F16, PATH_FUSED_F8 = 0, 1, 2
 ENCODING_TRI12, ENCODING_SINCOS6, ENCODING_UV_SINCOS16 = 0, 1, 2
 LOSS_RELATIVE_L2_LUMINANCE, LOSS_L2 = 0, 1
 INIT_KAIMING_NORMAL, INIT_XAVIER_UNIFORM = 0, 1
@@ -100,6 +100,7 @@ def load_library():
     L.nrc_frame.argtypes = [P, P, P, U32, P, P, P, U32, C.c_int, P]
     L.nrc_get_state.argtypes = [P, P, C.POINTER(OptimizerState), P, P]
     L.nrc_set_state.argtypes = [P, P, C.POINTER(OptimizerState)]
+    L.nrc_get_fp8_scales.argtypes = [P, P, P, P]
     L.nrc_encode.argtypes = [C.c_int, P, U32, P, P]
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
@@ -279,6 +280,16 @@ class NRCState:
         return {"m": entries[:, 0].copy(), "v": entries[:, 1].copy(), "weight": entries[:, 2].copy(),
                 "ema_weight": entries[:, 3].copy(), "weights_f16": w16.view(np.float16).copy(),
                 "use_weights_f16": u16.view(np.float16).copy(), "state": st.as_dict()}
+
+    def get_fp8_scales(self) -> dict:
+        """The power-of-two scales of the last FP8 inference (synchronous): layer_scales[l] = C_l,
+        l = 0..H (activations enter layer l as C_l a_l, weights as (C_{l+1}/C_l) W_l), out_scale,
+        and the number of FP8 inferences in the activation history."""
+        sc = np.zeros(self.config.hidden_layers + 1, np.float32)
+        o = C.c_float(0.0)
+        f = C.c_uint32(0)
+        _check(self._lib.nrc_get_fp8_scales(self._h, sc.ctypes.data_as(C.c_void_p), C.byref(o), C.byref(f)))
+        return {"layer_scales": sc, "out_scale": float(o.value), "frames": int(f.value)}
 
     def get_optimizer_state(self) -> dict:
         """The scalar optimizer state only (t, beta^t, alpha^t, loss scale, last loss); synchronous."""

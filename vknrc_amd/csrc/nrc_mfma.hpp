@@ -69,6 +69,57 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // --------------------------------------------------------------- encoders
+// The inference kernels round the features to FP16 (or e4m3) right away, so they
+// use shorter forms of two encoders than the FP32 restatements in nrc_device.hpp,
+// equal in exact arithmetic and within a few FP32 ulps of them:
+//   * one-blob (NRCOneBlob4Encode, NRCRecord.glsl:51-63): per knot k = 0..4,
+//     c_k = clamp(0.5 + u (15/16 - 5/8 u^2 + 3/16 u^4), 0, 1) with u = k - 4x,
+//     which is (15/16) u (1 - (2/3) u^2 + (1/5) u^4) + 0.5 in Horner form
+//     (6 instructions a knot instead of 11);
+//   * SINCOS6: sin/cos(2^k pi p) for k = 0..5 from one polynomial evaluation at
+//     k = 0 and five double-angle steps (largest error 1.3e-5 at k = 5, a twentieth
+//     of an FP16 half-ulp at 1);
+//   * TRI12 (_nrc_tri, NRCRecord.glsl:65-68): with a/2 = 2^(k-1) p - 1/4, the GLSL
+//     2 |mod(a, 2) - 1| - 1 is |4 fract(a/2) - 2| - 1, the same value (every step
+//     scales by a power of two or rounds once in both) except where fract clamps
+//     a result that rounds to 1 down to 1 - 2^-24.
+__device__ __forceinline__ void oneblob4_knots(float x, float *o) {
+	const float x4 = 4.0f * x;
+	float c[5];
+#pragma unroll
+	for (int k = 0; k < 5; ++k) {
+		const float u = (float)k - x4;
+		const float u2 = u * u;
+		float t = fmaf(u2, 0.1875f, -0.625f);
+		t = fmaf(u2, t, 0.9375f);
+		c[k] = fminf(fmaxf(fmaf(u, t, 0.5f), 0.0f), 1.0f);
+	}
+#pragma unroll
+	for (int b = 0; b < 4; ++b) o[b] = c[b + 1] - c[b];
+}
+
+template <int ENC> __device__ __forceinline__ void axis12_infer(float p, float *o) {
+	if constexpr (ENC == NRC_ENCODING_SINCOS6) {
+		float sv, cv;
+		sincospi_poly(p, &sv, &cv);
+		o[0] = sv;
+		o[6] = cv;
+#pragma unroll
+		for (int k = 1; k < 6; ++k) {
+			const float ns = 2.0f * sv * cv, nc = 1.0f - 2.0f * sv * sv;
+			sv = ns;
+			cv = nc;
+			o[k] = sv;
+			o[6 + k] = cv;
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < 12; ++k) {
+			const float h = __builtin_amdgcn_fractf(fmaf(p, 0.5f * (float)(1 << k), -0.25f));
+			o[k] = fabsf(fmaf(h, 4.0f, -2.0f)) - 1.0f;
+		}
+	}
+}
 // Emits the 16 feature quads (features 4p..4p+3) of one record; Emit(p, f0..f3).
 template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(const float *rec, Emit &&emit) {
 	if constexpr (ENC == NRC_ENCODING_UV_SINCOS16) {
@@ -97,21 +148,21 @@ template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(c
 #pragma unroll
 		for (int a = 0; a < 3; ++a) {
 			float t[12];
-			axis12<ENC>(rec[a], t);
+			axis12_infer<ENC>(rec[a], t);
 			emit(3 * a + 0, t[0], t[1], t[2], t[3]);
 			emit(3 * a + 1, t[4], t[5], t[6], t[7]);
 			emit(3 * a + 2, t[8], t[9], t[10], t[11]);
 		}
 		float b[4];
-		oneblob4(rec[3], b);
+		oneblob4_knots(rec[3], b);
 		emit(9, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[4], b);
+		oneblob4_knots(rec[4], b);
 		emit(10, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[5], b);
+		oneblob4_knots(rec[5], b);
 		emit(11, b[0], b[1], b[2], b[3]);
-		oneblob4(rec[6], b);
+		oneblob4_knots(rec[6], b);
 		emit(12, b[0], b[1], b[2], b[3]);
-		oneblob4(1.0f - expf(-rec[7]), b);
+		oneblob4_knots(1.0f - expf(-rec[7]), b);
 		emit(13, b[0], b[1], b[2], b[3]);
 		emit(14, rec[8], rec[9], rec[10], rec[11]);
 		emit(15, rec[12], rec[13], 1.0f, 1.0f);
@@ -146,13 +197,13 @@ template <int ENC, int P, typename Emit> __device__ __forceinline__ void encode_
 		emit(8 * a + 4 + k0 / 4, c[0], c[1], c[2], c[3]);
 	} else if constexpr (P < 3) {
 		float t[12];
-		axis12<ENC>(rec[P], t);
+		axis12_infer<ENC>(rec[P], t);
 		emit(3 * P + 0, t[0], t[1], t[2], t[3]);
 		emit(3 * P + 1, t[4], t[5], t[6], t[7]);
 		emit(3 * P + 2, t[8], t[9], t[10], t[11]);
 	} else if constexpr (P < 8) {
 		float b[4];
-		oneblob4(P == 7 ? 1.0f - expf(-rec[7]) : rec[P], b);
+		oneblob4_knots(P == 7 ? 1.0f - expf(-rec[7]) : rec[P], b);
 		emit(6 + P, b[0], b[1], b[2], b[3]);
 	} else {
 		emit(14, rec[8], rec[9], rec[10], rec[11]);

@@ -14,7 +14,11 @@ The line also carries the 16-cache workload (`group16`): 16 independent caches
 with distinct seeded weights and light sets, per-frame query counts
 log-uniform in [65,536, 2,073,600] and training batches uniform in
 {4096, 8192, 16384}, timed as sequential per-cache frames, as one grouped frame
-and as a replayed HIP graph of the grouped frame.
+and as a replayed HIP graph of the grouped frame; the FP8 inference (`fp8`);
+and the served workload (`served`): render sessions arriving as a Poisson
+process of 20/s at 720p-2160p, 300 frames of inference + one training step
+each, admitted by an HBM budget and batched continuously into grouped launches
+(p50/p99 frame latency, admission waits, aggregate queries/s).
 """
 from __future__ import annotations
 
@@ -103,10 +107,7 @@ def group_workload(torch, v, wl, q_host, frames: int) -> dict:
             e["cache"].frame(e["queries"], e["rgb"], e["batches"], max_queries=e["max_queries"],
                              max_batch_records=e["max_batch_records"])
 
-    graph = v.GroupGraph(entries)
-    modes = {"sequential": sequential, "grouped": lambda: v.group_frame(entries), "graph": graph.launch}
-    out = {}
-    for name, fn in modes.items():
+    def timed(fn):
         for _ in range(3):
             fn()
         torch.cuda.synchronize()
@@ -114,7 +115,15 @@ def group_workload(torch, v, wl, q_host, frames: int) -> dict:
         for _ in range(frames):
             fn()
         torch.cuda.synchronize()
-        out[name] = (time.perf_counter() - t0) * 1e3 / frames
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    out = {"sequential": timed(sequential)}     # default schedules: each cache's step fills the GPU alone
+    from vknrc_amd.autotune import tune_grad_chunks, tune_infer_workgroups
+    tuned_infer = tune_infer_workgroups(torch, entries)
+    tuned_grad = tune_grad_chunks(torch, entries)
+    out["grouped"] = timed(lambda: v.group_frame(entries))
+    graph = v.GroupGraph(entries)
+    out["graph"] = timed(graph.launch)
     graph.close()
     total_q = int(nq.sum())
     total_t = int(4 * nb.sum())
@@ -122,7 +131,31 @@ def group_workload(torch, v, wl, q_host, frames: int) -> dict:
             "sequential_ms_per_frame": out["sequential"], "grouped_ms_per_frame": out["grouped"],
             "graph_ms_per_frame": out["graph"], "graph_queries_per_s": total_q / (out["graph"] / 1e3),
             "grouped_vs_sequential": out["sequential"] / out["grouped"],
-            "graph_vs_sequential": out["sequential"] / out["graph"]}
+            "graph_vs_sequential": out["sequential"] / out["graph"],
+            "autotuned": {"grad_chunks": tuned_grad["best"],
+                          "grad_ms_per_step": {str(k): round(x, 4) for k, x in tuned_grad["ms_per_step"].items()},
+                          "infer_workgroups_per_cu": tuned_infer["best"],
+                          "infer_ms": {str(k): round(x, 4) for k, x in tuned_infer["ms_per_launch"].items()}}}
+
+
+def served_workload(torch, v, wl, dq, drec, sessions: int, budget_gib: float) -> dict:
+    """Render sessions with Poisson arrivals (mean 20/s) at 720p/1080p/1440p/2160p (weights
+    .3/.4/.2/.1), 300 frames each of full-resolution inference + one 16,384-record training step,
+    admitted by an HBM budget and run by continuous batching into grouped launches."""
+    from vknrc_amd.serving import Server, poisson_sessions
+    rng = np.random.default_rng(wl.config_seed(4))
+    specs = poisson_sessions(sessions, 20.0, rng, frames=300, seed0=wl.config_seed(4, 1) % 100000)
+    pool = torch.cat([dq] * 4)[:3840 * 2160]
+
+    def make_config(spec):
+        return v.NRCConfig(seed=spec.seed, encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE,
+                           loss_scale=128.0, dynamic_loss_scale=True)
+
+    srv = Server(torch, pool, drec, make_config, budget_bytes=int(budget_gib * (1 << 30)), max_sessions=256)
+    rep = srv.run_realtime(specs)
+    rep["arrival_rate_per_s"] = 20.0
+    rep["hbm_budget_GiB"] = budget_gib
+    return rep
 
 
 def main() -> int:
@@ -134,6 +167,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-frames", type=int, default=30, help="timed frames of the 16-cache workload (0: skip)")
+    ap.add_argument("--serve-sessions", type=int, default=60,
+                    help="sessions of the served workload (0: skip)")
+    ap.add_argument("--serve-budget-gib", type=float, default=48.0, help="HBM budget of the served workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run one elementwise copy (torch.neg) of 256 MiB fp32 -> 256 MiB, the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -270,6 +306,9 @@ def main() -> int:
     if args.group_frames > 0:
         log("16-cache workload ...")
         res["group16"] = group_workload(torch, v, wl, q, args.group_frames)
+    if args.serve_sessions > 0:
+        log("served workload ...")
+        res["served"] = served_workload(torch, v, wl, dq, drec, args.serve_sessions, args.serve_budget_gib)
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

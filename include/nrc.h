@@ -182,6 +182,26 @@ nrc_status nrc_get_state(nrc_cache *cache, nrc_optimizer_entry *host_entries, nr
 nrc_status nrc_set_state(nrc_cache *cache, const nrc_optimizer_entry *host_entries,
                          const nrc_optimizer_state *host_state);
 
+/* Device bytes a cache with this description allocates (each allocation
+ * rounded up to 4 KiB; the plain path's training scratch, allocated on its first
+ * plain training step, counted when path is PLAIN_F32).  For admission control
+ * by an HBM budget; no device call.  The reference sizes a cache's record
+ * buffers with VkNRCState::GetEvalRecordBufferSize / GetBatchTrainRecordBufferSize
+ * (VkNRCState.cpp:34-37) and has no admission control. */
+nrc_status nrc_cache_footprint(const nrc_cache_desc *desc, nrc_path path, uint64_t *device_bytes);
+
+/* Launch schedule of the cache's fused training steps: `chunks` 64-record chunks
+ * per gradient workgroup, 1..16 (0, the default: chosen from the batch size so
+ * that one cache's step fills the GPU).  The chunking fixes the summation order
+ * of the batch-reduced dW, so a cache trains to the same bits alone and in any
+ * group with one setting; the autotuner sets it per workload.  No counterpart in
+ * the reference, which runs 128-invocation workgroups (nrc_gradient.comp:5-6) and
+ * sums dW with float atomics (NN_nv.glsl:311-313, 361-363). */
+nrc_status nrc_cache_set_grad_chunks(nrc_cache *cache, uint32_t chunks);
+/* Persistent inference workgroups per CU, 1..8 (default 2), for every later
+ * launch in the process.  Inference results do not depend on it. */
+nrc_status nrc_set_infer_workgroups_per_cu(uint32_t workgroups);
+
 /* The FP8 scales the cache's last FP8 inference ran with (synchronous):
  * layer_scales[l] = C_l for l = 0..hidden_layers (the activations a_l enter
  * layer l as C_l a_l in e4m3, layer l's weights as (C_{l+1} / C_l) W_l), and

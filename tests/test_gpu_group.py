@@ -124,3 +124,37 @@ def test_group_rejects_a_cache_twice():
     with pytest.raises(v.NRCError):
         v.group_train([entries[0], entries[1], entries[0]], 0)
     v.group_infer([entries[0], entries[0]])  # inference changes no state: allowed
+
+
+@pytest.mark.parametrize("chunks", [2, 8, 16])
+def test_grad_chunks_give_the_same_bits_alone_and_grouped(chunks):
+    """A non-default gradient chunking is a property of the cache: grouped and single frames still
+    agree bit for bit.  Against the default chunking only the FP32 summation order of dW differs;
+    Adam's normalised step can turn that noise into up to about one learning-rate step on weights
+    whose gradient is near zero, so the weights are compared as a whole (relative L2)."""
+    queries, batches = [20000, 3000], [16384, 8192, 4096, 1000]
+    n = 6
+    _, grouped = build(n, queries, batches, 17)
+    _, alone = build(n, queries, batches, 17)
+    _, default = build(n, queries, batches, 17)
+    for e in grouped + alone:
+        e["cache"].set_grad_chunks(chunks)
+    for _ in range(2):
+        v.group_frame(grouped)
+        v.group_frame(default)
+        for e in alone:
+            single_frame(e)
+    assert_same(grouped, alone)
+    for a, d in zip(grouped, default):
+        wa, wd = a["cache"].get_state()["weight"], d["cache"].get_state()["weight"]
+        assert np.linalg.norm(wa - wd) <= 2e-3 * np.linalg.norm(wd)
+
+
+def test_grad_chunks_are_validated():
+    st = v.NRCState(v.NRCConfig())
+    with pytest.raises(v.NRCError):
+        st.set_grad_chunks(17)
+    st.set_grad_chunks(0)
+    with pytest.raises(v.NRCError):
+        v.set_infer_workgroups_per_cu(0)
+    v.set_infer_workgroups_per_cu(2)

@@ -22,6 +22,7 @@ struct nrc_cache {
 	_Float16 *w16 = nullptr, *use16 = nullptr;
 	_Float16 *wf = nullptr, *wtf = nullptr; // fragment-major copies for the fused gradient
 	_Float16 *usef = nullptr;               // fragment-major use-weights for the fused inference
+	uint32_t grad_chunks = 0;               // 64-record chunks per gradient workgroup (0: automatic)
 	uint8_t *w8 = nullptr;                  // fragment-major e4m3 use-weights (FP8 inference)
 	nrc::Fp8State *f8 = nullptr;            // FP8 scales and activation history
 	float *w32 = nullptr, *use32 = nullptr;
@@ -236,7 +237,7 @@ nrc_status check_train(nrc_cache *c, const nrc_train_record *d_records, uint32_t
 	const uint32_t clamp = c->desc.train_batch_size;
 	const uint32_t n = max_count < clamp ? max_count : clamp;
 	const uint32_t need = path == NRC_PATH_PLAIN_F32 ? (n + nrc::kPlainChunk - 1) / nrc::kPlainChunk
-	                                                 : nrc::plan_fused_gradient(n).blocks;
+	                                                 : nrc::plan_fused_gradient(n, c->grad_chunks).blocks;
 	if (need > c->part_cap) return fail(NRC_ERROR_CAPACITY, "partial buffer too small");
 	return NRC_OK;
 }
@@ -258,7 +259,7 @@ nrc_status train_step(nrc_cache *c, const nrc_train_record *d_records, const uin
 		a.ldn = c->ldn;
 		e = nrc::launch_plain_gradient(a, &nparts, s);
 	} else {
-		const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+		const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n, c->grad_chunks);
 		e = nrc::launch_fused_gradient(a, plan, s);
 		nparts = plan.blocks;
 	}
@@ -319,6 +320,52 @@ nrc_status nrc_init_weights(const nrc_cache_desc *desc, uint64_t seed, float *ho
 	return NRC_OK;
 }
 
+} // extern "C"
+
+namespace {
+// Device bytes of each allocation of a cache with this description (one table
+// for nrc_cache_create and nrc_cache_footprint).
+struct CacheSizes {
+	size_t entries, w16, wf, wtf, usef, w8, f8, use16, w32, use32, state, partials, plain_scratch;
+	explicit CacheSizes(const nrc_cache_desc &d) {
+		const size_t wc = nrc_weight_count(d.hidden_layers);
+		const int H = (int)d.hidden_layers;
+		const size_t pstride = ((wc + 1) + 63u) & ~size_t(63);
+		const size_t part_cap = (d.train_batch_size + nrc::kFusedSmallRecords - 1) / nrc::kFusedSmallRecords;
+		entries = sizeof(nrc_optimizer_entry) * wc;
+		w16 = use16 = 2 * wc;
+		wf = usef = 2 * (size_t)nrc::fwd_frag_halves(H);
+		wtf = 2 * (size_t)nrc::bwd_frag_halves(H);
+		w8 = (size_t)nrc::fwd_frag_halves(H);
+		f8 = sizeof(nrc::Fp8State);
+		w32 = use32 = 4 * wc;
+		state = sizeof(DeviceState);
+		partials = sizeof(float) * pstride * part_cap;
+		const size_t ldn = (d.train_batch_size + 63u) & ~size_t(63);
+		plain_scratch = (2 * (size_t)(d.hidden_layers + 1) * 64 * ldn + ldn) * sizeof(float);
+	}
+	// what the allocator hands out: each allocation rounded up to 4 KiB
+	static size_t page(size_t b) { return (b + 4095) & ~size_t(4095); }
+	uint64_t total(bool plain) const {
+		const size_t all[] = {entries, w16, wf, wtf, usef, w8, f8, use16, w32, use32, state, partials};
+		uint64_t t = 0;
+		for (size_t b : all) t += page(b);
+		if (plain) t += 2 * page(plain_scratch / 2);
+		return t;
+	}
+};
+} // namespace
+
+extern "C" {
+
+nrc_status nrc_cache_footprint(const nrc_cache_desc *desc, nrc_path path, uint64_t *device_bytes) {
+	std::string why;
+	if (!device_bytes) return fail(NRC_ERROR_INVALID_ARGUMENT, "device_bytes is NULL");
+	if (!valid_desc(desc, why)) return fail(NRC_ERROR_INVALID_ARGUMENT, why);
+	*device_bytes = CacheSizes(*desc).total(path == NRC_PATH_PLAIN_F32);
+	return NRC_OK;
+}
+
 nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	std::string why;
 	if (!out) return fail(NRC_ERROR_INVALID_ARGUMENT, "out_cache is NULL");
@@ -330,19 +377,20 @@ nrc_status nrc_cache_create(const nrc_cache_desc *desc, nrc_cache **out) {
 	c->pstride = ((c->wc + 1) + 63u) & ~63u;
 	// one partial row per 64 records covers every fused-gradient plan (and the plain path's 512)
 	c->part_cap = (desc->train_batch_size + nrc::kFusedSmallRecords - 1) / nrc::kFusedSmallRecords;
+	const CacheSizes sz(*desc);
 	hipError_t e = hipGetDevice(&c->device);
-	if (e == hipSuccess) e = hipMalloc(&c->entries, sizeof(nrc_optimizer_entry) * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->w16, 2 * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->wf, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
-	if (e == hipSuccess) e = hipMalloc(&c->wtf, 2 * (size_t)nrc::bwd_frag_halves((int)desc->hidden_layers));
-	if (e == hipSuccess) e = hipMalloc(&c->usef, 2 * (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
-	if (e == hipSuccess) e = hipMalloc(&c->w8, (size_t)nrc::fwd_frag_halves((int)desc->hidden_layers));
-	if (e == hipSuccess) e = hipMalloc(&c->f8, sizeof(nrc::Fp8State));
-	if (e == hipSuccess) e = hipMalloc(&c->use16, 2 * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->w32, 4 * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->use32, 4 * c->wc);
-	if (e == hipSuccess) e = hipMalloc(&c->state, sizeof(DeviceState));
-	if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(float) * (size_t)c->pstride * c->part_cap);
+	if (e == hipSuccess) e = hipMalloc(&c->entries, sz.entries);
+	if (e == hipSuccess) e = hipMalloc(&c->w16, sz.w16);
+	if (e == hipSuccess) e = hipMalloc(&c->wf, sz.wf);
+	if (e == hipSuccess) e = hipMalloc(&c->wtf, sz.wtf);
+	if (e == hipSuccess) e = hipMalloc(&c->usef, sz.usef);
+	if (e == hipSuccess) e = hipMalloc(&c->w8, sz.w8);
+	if (e == hipSuccess) e = hipMalloc(&c->f8, sz.f8);
+	if (e == hipSuccess) e = hipMalloc(&c->use16, sz.use16);
+	if (e == hipSuccess) e = hipMalloc(&c->w32, sz.w32);
+	if (e == hipSuccess) e = hipMalloc(&c->use32, sz.use32);
+	if (e == hipSuccess) e = hipMalloc(&c->state, sz.state);
+	if (e == hipSuccess) e = hipMalloc(&c->partials, sz.partials);
 
 	if (e != hipSuccess) {
 		free_cache(c);
@@ -409,6 +457,19 @@ nrc_status nrc_infer(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d
 	default: return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	}
 	if (e != hipSuccess) return hip_fail(e, "nrc_infer launch");
+	return NRC_OK;
+}
+
+nrc_status nrc_cache_set_grad_chunks(nrc_cache *c, uint32_t chunks) {
+	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
+	if (chunks > (uint32_t)nrc::kFusedMaxChunks) return fail(NRC_ERROR_INVALID_ARGUMENT, "chunks > 16");
+	c->grad_chunks = chunks;
+	return NRC_OK;
+}
+
+nrc_status nrc_set_infer_workgroups_per_cu(uint32_t workgroups) {
+	if (workgroups < 1 || workgroups > 8) return fail(NRC_ERROR_INVALID_ARGUMENT, "workgroups per CU outside 1..8");
+	nrc::set_infer_wg_per_cu(workgroups);
 	return NRC_OK;
 }
 
@@ -640,7 +701,7 @@ static nrc_status group_train_impl(const nrc_group_item *items, uint32_t n_items
 			nrc_cache *c = it.cache;
 			const nrc::TrainArgs a = train_args(c, records(it), counts(it), it.max_batch_records);
 			const uint32_t n = it.max_batch_records < a.clamp ? it.max_batch_records : a.clamp;
-			const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n);
+			const nrc::FusedGradPlan plan = nrc::plan_fused_gradient(n, c->grad_chunks);
 			grad.item[grad.n_items++] = nrc::grad_item(a, plan);
 			adam.item[adam.n_items++] = adam_args(c, counts(it), it.max_batch_records, plan.blocks, flags);
 		}

@@ -18,6 +18,7 @@ namespace {
 
 constexpr float kFp8Max = 448.0f;
 constexpr float kFp8Target = 224.0f;
+constexpr uint32_t kFp8SampleStride = 8; // tiles whose activation maxima feed the scales
 
 __device__ __forceinline__ float4_t mfma8(long a, long b, float4_t c) {
 	return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
@@ -45,14 +46,16 @@ __device__ __forceinline__ float max4(float m, float4_t v) {
 
 // Same schedule as fused_infer_kernel (grouped, persistent, contiguous tile
 // ranges per wave, encode into the wave's LDS slot, records of the next tile in
-// flight), with 8-byte A and B fragments.  The first tile of every cache segment
-// a wave runs also records the largest pre-saturation activation of each layer;
-// the wave's maxima go to its block's slot of the cache's FP8 state.
+// flight), with 8-byte A and B fragments.  Every kFp8SampleStride-th tile of a
+// cache also records the largest pre-saturation activation of each layer (a
+// sample that does not depend on how the tiles are split among waves, so a
+// cache gets the same scales alone and in a group); the wave's maxima go to its
+// block's slot of the cache's FP8 state.
 template <int H, int ENC, int TILES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void fp8_infer_kernel(const InferGroup grp) {
 	__shared__ __attribute__((aligned(16))) uint64_t sX[4][8 * 64];
 
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int g = lane >> 4, r = lane & 15;
 	const uint32_t nW = gridDim.x * 4, gw = blockIdx.x * 4 + wave;
 	const uint32_t t0 = (uint32_t)((uint64_t)grp.total_tiles * gw / nW);
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 		const uint32_t b0 = it.tile_begin, b1 = b0 + (it.max_count + 63) / 64;
 		if (b1 <= t0) continue;
 		if (b0 >= t1) break;
-		const uint32_t count = effective_count(it.d_count, it.max_count, it.max_count);
+		const uint32_t count = __builtin_amdgcn_readfirstlane(effective_count(it.d_count, it.max_count, it.max_count));
 		const uint32_t lt0 = (t0 > b0 ? t0 : b0) - b0;
 		uint32_t lt1 = (t1 < b1 ? t1 : b1) - b0;
 		const uint32_t ntiles = (count + 63) / 64;
@@ -174,18 +177,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 			wave_lds_sync();
 		};
 
-		// ping-pong record buffers as in fused_infer_kernel; the segment's first
-		// tile also records the activation maxima
+		// ping-pong record buffers as in fused_infer_kernel; sampled tiles also
+		// record the activation maxima
 		float ra[kQueryFloats], rb[kQueryFloats];
 		load_query(lt0, ra);
 		for (uint32_t t = lt0; t < lt1; t += 2) {
 			load_query(t + 1, rb);
 			encode_tile(t * 64, ra);
-			mlp_tile(t * 64, t == lt0);
+			mlp_tile(t * 64, t % kFp8SampleStride == 0);
 			if (t + 1 >= lt1) break;
 			load_query(t + 2, ra);
 			encode_tile((t + 1) * 64, rb);
-			mlp_tile((t + 1) * 64, false);
+			mlp_tile((t + 1) * 64, (t + 1) % kFp8SampleStride == 0);
 		}
 		// the wave's maxima into its block's slot (non-negative floats order as their bits)
 #pragma unroll

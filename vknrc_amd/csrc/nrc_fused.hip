@@ -12,6 +12,8 @@
 
 #include "nrc_mfma.hpp"
 
+#include <atomic>
+
 namespace nrc {
 
 namespace {
@@ -785,15 +787,20 @@ template <int H> hipError_t infer_h(int enc, const InferGroup &grp, hipStream_t 
 
 } // namespace
 
+static std::atomic<uint32_t> g_infer_wg_per_cu{0};
+
 uint32_t infer_wg_per_cu() {
-	static int v = 0;
+	uint32_t v = g_infer_wg_per_cu.load(std::memory_order_relaxed);
 	if (v == 0) {
 		const char *e = getenv("VKNRC_INFER_WG_PER_CU");
-		v = e ? atoi(e) : 2;
-		if (v < 1) v = 1;
+		const int x = e ? atoi(e) : 2;
+		v = x < 1 ? 1u : (uint32_t)x;
+		g_infer_wg_per_cu.store(v, std::memory_order_relaxed);
 	}
-	return (uint32_t)v;
+	return v;
 }
+
+void set_infer_wg_per_cu(uint32_t v) { g_infer_wg_per_cu.store(v, std::memory_order_relaxed); }
 
 int device_cu_count() {
 	static int cus = 0;
@@ -845,12 +852,12 @@ static int env_int(const char *name, int dflt) {
 // One 64-record chunk per workgroup up to about one workgroup per CU, then
 // several chunks per workgroup (dW summed in registers) so that the partial rows
 // stay about one per CU; VKNRC_GRAD_CHUNKS forces the chunk count.
-FusedGradPlan plan_fused_gradient(uint32_t n) {
+FusedGradPlan plan_fused_gradient(uint32_t n, uint32_t chunks) {
 	static const int forced = env_int("VKNRC_GRAD_CHUNKS", 0);
 	FusedGradPlan p;
 	const uint32_t nchunks = (n + kFusedSmallRecords - 1) / kFusedSmallRecords;
 	const uint32_t cus = (uint32_t)device_cu_count();
-	uint32_t c = forced > 0 ? (uint32_t)forced : (nchunks + cus - 1) / cus;
+	uint32_t c = chunks > 0 ? chunks : forced > 0 ? (uint32_t)forced : (nchunks + cus - 1) / cus;
 	if (c < 1) c = 1;
 	if (c > kFusedMaxChunks) c = kFusedMaxChunks;
 	p.chunks = c;

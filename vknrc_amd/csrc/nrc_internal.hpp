@@ -158,8 +158,10 @@ hipError_t launch_fp8_infer_group(int hidden, int encoding, InferGroup &grp, con
 // Forward over n pre-encoded fp16 input rows with fragment-major weights wf (no clamp).
 hipError_t launch_fused_forward_f16(int hidden, const _Float16 *inputs, uint32_t n, const _Float16 *wf, float *out,
                                     hipStream_t stream);
-// Workgroups and chunking of the fused gradient for a step of n records.
-FusedGradPlan plan_fused_gradient(uint32_t n);
+// Workgroups and chunking of the fused gradient for a step of n records: chunks
+// 64-record chunks a workgroup (0: from n and the CU count).  The chunking fixes
+// the summation order of dW, so a cache keeps one setting alone and in groups.
+FusedGradPlan plan_fused_gradient(uint32_t n, uint32_t chunks = 0);
 hipError_t launch_fused_gradient(const TrainArgs &a, const FusedGradPlan &plan, hipStream_t stream);
 // One launch over a group sharing (hidden, encoding, loss); fills block_begin and
 // total_blocks.
@@ -171,8 +173,10 @@ hipError_t launch_fused_gradient_f16(int hidden, const _Float16 *inputs, const f
                                      uint32_t *nparts, hipStream_t stream);
 
 int device_cu_count();
-// Persistent inference workgroups per CU (VKNRC_INFER_WG_PER_CU overrides 2).
+// Persistent inference workgroups per CU (default 2, VKNRC_INFER_WG_PER_CU or
+// set_infer_wg_per_cu override; results do not depend on it).
 uint32_t infer_wg_per_cu();
+void set_infer_wg_per_cu(uint32_t v);
 // One-time launch set-up of the fused kernels a cache of this kind uses (kernel
 // attributes, device properties), done at cache creation so that no launch
 // performs it under stream capture.

@@ -101,6 +101,9 @@ def load_library():
     L.nrc_get_state.argtypes = [P, P, C.POINTER(OptimizerState), P, P]
     L.nrc_set_state.argtypes = [P, P, C.POINTER(OptimizerState)]
     L.nrc_get_fp8_scales.argtypes = [P, P, P, P]
+    L.nrc_cache_footprint.argtypes = [C.POINTER(CacheDesc), C.c_int, P]
+    L.nrc_cache_set_grad_chunks.argtypes = [P, U32]
+    L.nrc_set_infer_workgroups_per_cu.argtypes = [U32]
     L.nrc_encode.argtypes = [C.c_int, P, U32, P, P]
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
@@ -199,6 +202,19 @@ def init_weights(config: NRCConfig, seed: int | None = None) -> np.ndarray:
     return w
 
 
+def set_infer_workgroups_per_cu(workgroups: int) -> None:
+    """Persistent inference workgroups per CU for later launches (results do not depend on it)."""
+    _check(load_library().nrc_set_infer_workgroups_per_cu(int(workgroups)))
+
+
+def cache_footprint(config: NRCConfig, path: int = PATH_FUSED_F16) -> int:
+    """Device bytes a cache built from ``config`` allocates (host-only; nrc_cache_footprint)."""
+    d = config.to_c()
+    b = C.c_uint64(0)
+    _check(load_library().nrc_cache_footprint(C.byref(d), int(path), C.byref(b)))
+    return int(b.value)
+
+
 class NRCState:
     """One neural radiance cache: weights, use-weights and optimizer state on the device."""
 
@@ -280,6 +296,11 @@ class NRCState:
         return {"m": entries[:, 0].copy(), "v": entries[:, 1].copy(), "weight": entries[:, 2].copy(),
                 "ema_weight": entries[:, 3].copy(), "weights_f16": w16.view(np.float16).copy(),
                 "use_weights_f16": u16.view(np.float16).copy(), "state": st.as_dict()}
+
+    def set_grad_chunks(self, chunks: int) -> None:
+        """64-record chunks per gradient workgroup of this cache's fused training steps (0: automatic)."""
+        _check(self._lib.nrc_cache_set_grad_chunks(self._h, int(chunks)))
+        self.grad_chunks = int(chunks)
 
     def get_fp8_scales(self) -> dict:
         """The power-of-two scales of the last FP8 inference (synchronous): layer_scales[l] = C_l,

@@ -293,7 +293,7 @@ def main() -> int:
         "train_step_ms_avg": float(train_ms.mean()) / BATCHES,
         "plain_f32_infer_queries_per_s": QUERIES / (plain_ms / 1e3),
         "fused_vs_plain_infer": (QUERIES / infer_avg_s) / (QUERIES / (plain_ms / 1e3)),
-        "roofline": {"bound": "mfma", "kernel": "fused_infer_kernel", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "fused_infer_pipe_kernel", "achieved": achieved,
                      "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_F16_TFLOPS,
                      "traffic": INFER_TRAFFIC_BYTES,
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},

@@ -212,16 +212,24 @@ template <int H> __global__ __launch_bounds__(256) void fp8_scale_kernel(const F
 	const int t = threadIdx.x;
 	__shared__ float red[kFp8Layers][256];
 
-	// 1. the last inference's activation maxima (scaled by its C_l), cleared for the next
-	float m[kFp8Layers];
+	// 1. the last inference's activation maxima (scaled by its C_l), cleared for the
+	// next; every load is issued before the first store and the first maximum
+	constexpr int kPer = kFp8MaxBlocks / 256;
+	float m[kFp8Layers], v[kPer][kFp8Layers];
 #pragma unroll
-	for (int l = 0; l < kFp8Layers; ++l) m[l] = 0.0f;
-	for (int b = t; b < kFp8MaxBlocks; b += 256)
+	for (int j = 0; j < kPer; ++j)
 #pragma unroll
-		for (int l = 0; l < kFp8Layers; ++l) {
-			m[l] = fmaxf(m[l], st->block_amax[b][l]);
-			st->block_amax[b][l] = 0.0f;
-		}
+		for (int l = 0; l < kFp8Layers; ++l) v[j][l] = st->block_amax[t + 256 * j][l];
+#pragma unroll
+	for (int l = 0; l < kFp8Layers; ++l) {
+		m[l] = v[0][l];
+#pragma unroll
+		for (int j = 1; j < kPer; ++j) m[l] = fmaxf(m[l], v[j][l]);
+	}
+#pragma unroll
+	for (int j = 0; j < kPer; ++j)
+#pragma unroll
+		for (int l = 0; l < kFp8Layers; ++l) st->block_amax[t + 256 * j][l] = 0.0f;
 #pragma unroll
 	for (int l = 0; l < kFp8Layers; ++l) red[l][t] = m[l];
 	__syncthreads();
@@ -253,14 +261,24 @@ template <int H> __global__ __launch_bounds__(256) void fp8_scale_kernel(const F
 		hmaxs[t] = nh == 0 ? 4.0f : h; // no history yet: a first guess, corrected after one inference
 	}
 
-	// 2. largest |W_l| per layer
+	// 2. largest |W_l| per layer (all loads in flight at once)
 	const float *w = it.use32;
+	{
+		constexpr int kRow = kWidth * kWidth / 256; // weights of one hidden layer per thread
+		float wv[H][kRow];
 #pragma unroll
-	for (int l = 0; l <= H; ++l) {
-		const int n = l < H ? kWidth * kWidth : kOut * kWidth;
-		float a = 0.0f;
-		for (int i = t; i < n; i += 256) a = fmaxf(a, fabsf(w[l * kWidth * kWidth + i]));
-		red[l][t] = a;
+		for (int l = 0; l < H; ++l)
+#pragma unroll
+			for (int j = 0; j < kRow; ++j) wv[l][j] = w[l * kWidth * kWidth + t + 256 * j];
+		const float wo = t < kOut * kWidth ? w[H * kWidth * kWidth + t] : 0.0f;
+#pragma unroll
+		for (int l = 0; l < H; ++l) {
+			float a = fabsf(wv[l][0]);
+#pragma unroll
+			for (int j = 1; j < kRow; ++j) a = fmaxf(a, fabsf(wv[l][j]));
+			red[l][t] = a;
+		}
+		red[H][t] = fabsf(wo);
 	}
 	__syncthreads();
 	for (int s = 128; s > 0; s >>= 1) {

@@ -387,8 +387,12 @@ template <int H, int ENC, int LOSS, bool F16IN, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void fused_grad_small_kernel(
     const GradGroup grp) {
 	// this workgroup's cache: the last item whose first block is at or below blockIdx.x
+	// (block_begin does not decrease, so k counts the later items starting at or below
+	// blockIdx.x; the unrolled count issues every table load at once)
 	uint32_t k = 0;
-	while (k + 1 < grp.n_items && blockIdx.x >= grp.item[k + 1].block_begin) ++k;
+#pragma unroll
+	for (uint32_t j = 1; j < (uint32_t)kGroupMax; ++j)
+		k += (j < grp.n_items && blockIdx.x >= grp.item[j].block_begin) ? 1u : 0u;
 	const GradItem &gi = grp.item[k];
 	const uint32_t blk = blockIdx.x - gi.block_begin;
 	const float *__restrict__ records = gi.records;
@@ -450,25 +454,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	float q[kQueryFloats];
 	uint4 fq[2];
 	float tg[3];
-	auto load_records = [&](uint32_t base) __attribute__((always_inline)) {
+	// the inputs this lane encodes and the targets of the record it runs, loaded
+	// separately: a multi-chunk workgroup loads the next chunk's inputs once the
+	// current ones are encoded and its targets once the loss is taken, so both
+	// loads are in flight during the rest of the chunk
+	auto load_inputs = [&](uint32_t base) __attribute__((always_inline)) {
 		const uint32_t n_enc = base + lane < last ? base + lane : last; // the record this lane encodes
-		const uint32_t n_run = base + col < last ? base + col : last;   // the record this lane runs
 		if constexpr (F16IN) {
 			const uint4 *src = reinterpret_cast<const uint4 *>(fin + (size_t)n_enc * kWidth + 16 * wave);
 			fq[0] = src[0];
 			fq[1] = src[1];
-#pragma unroll
-			for (int c = 0; c < 3; ++c) tg[c] = ftgt[(size_t)n_run * 3 + c];
 		} else {
 			const float *rp = records + (size_t)n_enc * kTrainFloats;
 #pragma unroll
 			for (int i = 0; i < kQueryFloats; ++i) q[i] = rp[i];
+		}
+	};
+	auto load_targets = [&](uint32_t base) __attribute__((always_inline)) {
+		const uint32_t n_run = base + col < last ? base + col : last; // the record this lane runs
+		if constexpr (F16IN) {
+#pragma unroll
+			for (int c = 0; c < 3; ++c) tg[c] = ftgt[(size_t)n_run * 3 + c];
+		} else {
 			const float *tp = records + (size_t)n_run * kTrainFloats + kQueryFloats;
 #pragma unroll
 			for (int c = 0; c < 3; ++c) tg[c] = tp[c];
 		}
 	};
-	load_records(base0);
+	load_inputs(base0);
+	load_targets(base0);
 	// the backward fragments go to LDS once per workgroup: a straight copy of 1 KB
 	// blocks, wave-instruction k putting lane i at byte 1024 k + 16 i
 	for (int k = wave; k < H * 8; k += 4)
@@ -503,10 +517,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	for (uint32_t ch = 0; ch < nch; ++ch) {
 		const uint32_t base = base0 + ch * (uint32_t)R;
 		if (base >= count) break; // uniform
-		if (ch > 0) {
-			load_records(base);
-			__syncthreads(); // the previous chunk's dW readers are done with the LDS buffers
-		}
+		if (ch > 0) block_lds_sync(); // the previous chunk's dW readers are done with the LDS buffers
 		if constexpr (MULTI) {
 			// recompute the lane-derived indices each chunk from an opaque copy of the
 			// lane id, so the LDS addresses built from them are not kept live across chunks
@@ -543,7 +554,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			}
 		}
 		if (ch == 0) __builtin_amdgcn_s_waitcnt(0); // this wave's fragment copies to LDS have landed
-		__syncthreads();
+		block_lds_sync();
+		if constexpr (MULTI)
+			if (ch + 1 < nch) load_inputs(base + R); // the next chunk's inputs (uniform branch)
 
 		// ---- forward for this wave's 16 records; a_l kept as B operands and written feature-major
 		// (the activations are not kept in registers for the backward: its ReLU masks
@@ -581,6 +594,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				gm = fmaxf(gm, fabsf(go[c]));
 			}
 		}
+		if constexpr (MULTI)
+			if (ch + 1 < nch) load_targets(base + R); // the next chunk's targets (uniform branch)
 		// The deltas of the workgroup share one power-of-two scale 2^k: k = 0 unless
 		// the largest output gradient reaches kDeltaLimit, where FP16 would overflow
 		// and force a skipped step.  Scaling by 2^k is exact, and the dW tiles undo it
@@ -588,7 +603,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 		for (int off = 32; off >= 1; off >>= 1) gm = fmaxf(gm, __shfl_xor(gm, off));
 		if (lane == 0) S.gmax[wave] = gm;
-		__syncthreads();
+		block_lds_sync();
 		float dscale = 1.0f, undo = 1.0f;
 		{
 			const float wmax = fmaxf(fmaxf(S.gmax[0], S.gmax[1]), fmaxf(S.gmax[2], S.gmax[3]));
@@ -633,7 +648,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 		for (int off = 32; off >= 1; off >>= 1) lv += __shfl_xor(lv, off);
 		if (lane == 0) S.lsum[wave] += lv;
-		__syncthreads();
+		block_lds_sync();
 
 		// ---- dW_l^T tiles of this wave over the chunk's records (K = the 64 records)
 #pragma unroll

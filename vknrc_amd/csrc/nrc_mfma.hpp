@@ -50,6 +50,15 @@ __device__ __forceinline__ half8_t pack_relu(float4_t lo, float4_t hi) {
 	return h;
 }
 
+// Workgroup barrier that orders LDS traffic only: the fences are restricted to
+// LDS, so the barrier waits on LDS counters and leaves global loads in flight
+// (for data exchanged through LDS; global memory needs __syncthreads).
+__device__ __forceinline__ void block_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Calls f(std::integral_constant<int, i>{}) for i = 0 .. N-1, unrolled at compile time.
 template <typename F, int... I> __device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
 	(f(std::integral_constant<int, I>{}), ...);

@@ -19,16 +19,57 @@ namespace {
 
 __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	// this block's cache: the last item whose first block is at or below blockIdx.x
+	// (block_begin does not decrease, so k counts the later items starting at or below
+	// blockIdx.x; the unrolled count issues every table load at once)
 	uint32_t k = 0;
-	while (k + 1 < grp.n_items && blockIdx.x >= grp.item[k + 1].block_begin) ++k;
+#pragma unroll
+	for (uint32_t j = 1; j < (uint32_t)kGroupMax; ++j)
+		k += (j < grp.n_items && blockIdx.x >= grp.item[j].block_begin) ? 1u : 0u;
 	const AdamArgs &a = grp.item[k];
 	const uint32_t blk = blockIdx.x - a.block_begin;
 	const int t = threadIdx.x;
 	DeviceState *st = a.state;
 	const int p = a.parity;
+	// The state, count, overflow flag and partial rows are all loaded before the
+	// first decision that depends on one, so a block waits on memory once for them:
+	// with few partial rows per cache the kernel is latency-bound.  (The optimizer
+	// entry is loaded where it is updated; moving that load changes which products
+	// the compiler contracts into FMAs in the update, and with them the last bit.)
 	const nrc_optimizer_state s0 = st->pub[p];
 	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
+	const uint32_t ovf = st->overflow[p];
 	const bool loss_block = blk == a.weight_count / 64u;
+	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 4*col4 .. +3
+	const int col4 = t & 15, slice = t >> 4;
+	const uint32_t w0 = blk * 64u;
+	const uint32_t i = w0 + (uint32_t)t;
+	const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
+	const size_t ps4 = a.pstride / 4;
+	float4 acc[16];
+#pragma unroll
+	for (int u = 0; u < 16; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	// 16 rows a round, all 16 loads in flight before the first add (rows past
+	// nparts load the last row and add as exact zeros, so the order of the sums is
+	// that of the rows)
+	const uint32_t lastrow = a.nparts - 1;
+	for (uint32_t q = slice; q < a.nparts; q += 256) {
+		float4 v[16];
+#pragma unroll
+		for (int u = 0; u < 16; ++u) {
+			const uint32_t row = q + 16 * u;
+			v[u] = src[(size_t)(row < lastrow ? row : lastrow) * ps4];
+		}
+#pragma unroll
+		for (int u = 0; u < 16; ++u) {
+			const bool in = q + 16 * u < a.nparts;
+			acc[u].x += in ? v[u].x : 0.0f;
+			acc[u].y += in ? v[u].y : 0.0f;
+			acc[u].z += in ? v[u].z : 0.0f;
+			acc[u].w += in ? v[u].w : 0.0f;
+		}
+	}
+	const bool updates = t < 64 && i < a.weight_count;
+
 	if (count == 0) { // nrc_optimize.comp:33-34: an empty batch changes nothing
 		if (blk == 0 && t == 0) {
 			st->pub[p ^ 1] = s0;
@@ -36,34 +77,10 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 		}
 		return;
 	}
-	const bool skip = a.p.dynamic_scale && (st->overflow[p] != 0);
-
-	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 4*col4 .. +3
-	const int col4 = t & 15, slice = t >> 4;
-	const uint32_t w0 = blk * 64u;
+	const bool skip = a.p.dynamic_scale && (ovf != 0);
 	__shared__ float4 red[16][16];
 	float4 G4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	if (!skip) {
-		const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
-		const size_t ps4 = a.pstride / 4;
-		float4 acc[16];
-#pragma unroll
-		for (int u = 0; u < 16; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		uint32_t q = slice;
-		for (; q + 240 < a.nparts; q += 256) {
-#pragma unroll
-			for (int u = 0; u < 16; ++u) {
-				const float4 v = src[(q + 16 * u) * ps4];
-				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
-			}
-		}
-#pragma unroll
-		for (int u = 0; u < 16; ++u) {
-			if (q + 16 * u < a.nparts) {
-				const float4 v = src[(q + 16 * u) * ps4];
-				acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
-			}
-		}
 		G4 = acc[0];
 #pragma unroll
 		for (int u = 1; u < 16; ++u) {
@@ -84,11 +101,10 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 		}
 		return;
 	}
-	const uint32_t i = w0 + (uint32_t)t;
 	const float b1 = a.p.beta1, b2 = a.p.beta2, al = a.p.ema_alpha;
 	const float B1 = s0.beta1_t * b1, B2 = s0.beta2_t * b2; // nrc_train_prepare.comp:23-27
 	const float A1 = s0.alpha_t, A = s0.alpha_t * al;
-	if (t < 64 && !skip && i < a.weight_count) {
+	if (updates && !skip) {
 		float G = 0.0f;
 #pragma unroll
 		for (int sl = 0; sl < 16; ++sl) G += rf[sl * 64 + t];

@@ -217,6 +217,46 @@ nrc_status nrc_encode(nrc_encoding encoding, const nrc_query *d_queries, uint32_
 /* NRCSphEncode (NRCRecord.glsl:47-49): d_dirs [n][3] -> d_uv [n][2]. */
 nrc_status nrc_sph_encode(const float *d_dirs, uint32_t n, float *d_uv, void *stream);
 
+/* ---- inference epilogue: screen composite and training-target back-fill
+ * (nrc_inference.comp:50-72).  Every query carries a destination code
+ * (NRCEvalRecord.dst, NRCRecord.glsl:12-33):
+ *   NRC_DST_INVALID                    nothing;
+ *   screen: ((x | y << 15) << 1)       pixel (x, y), x, y < 2^15;
+ *   train:  ((b | l << 2 | r << 16) << 1) | 1   records l..r of batch b < 4. */
+#define NRC_DST_INVALID 0xFFFFFFFFu
+/* EncodeNRCEvalDstScreen / EncodeNRCEvalDstTrain (NRCRecord.glsl:19-22); the
+ * fields are masked to their widths. */
+uint32_t nrc_dst_screen(uint32_t x, uint32_t y);
+uint32_t nrc_dst_train(uint32_t batch, uint32_t l, uint32_t r);
+
+/* The screen of the composite: bias_factor_r is an RGBA32F image
+ * [height][pitch] holding (bias rgb, factor r) and receives (color, 0);
+ * factor_gb is an RG32F image [height][pitch] (uBias_FactorR / uFactorGB,
+ * nrc_inference.comp:25-26).  Pitches are in pixels. */
+typedef struct nrc_screen {
+	float *d_bias_factor_r;
+	const float *d_factor_gb;
+	uint32_t width, height, pitch;
+} nrc_screen;
+
+/* For each query i < count (count = *d_count clamped to max_count, or
+ * max_count when d_count is NULL), with pred = d_rgb[i] (the output of
+ * nrc_infer, already clamped at 0):
+ *   screen code: bias_factor_r[y][x] = (bias + (factor_r, factor_gb[y][x]) * pred, 0);
+ *   train code:  for j = l .. r: d_batches[b * batch_stride + j].target
+ *                += d_train_factors[(b * batch_stride + j) * 3 ..] * pred
+ *                (the reference's NRCTrainRecord bias, which nrc_train reads as
+ *                the target; its factor travels in d_train_factors).
+ * Each product and sum is rounded separately (no FMA contraction).  Pixels
+ * outside width x height and records j >= batch_stride are skipped, as a robust
+ * image store or buffer write would drop them.  Destinations of different
+ * queries must not overlap (true of the reference's path tracer, which gives
+ * every eval record its own pixel or record range).  screen or d_batches may be
+ * NULL when no code of that kind occurs. */
+nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32_t *d_count, uint32_t max_count,
+                         const nrc_screen *screen, nrc_train_record *d_batches, const float *d_train_factors,
+                         uint32_t batch_stride, void *stream);
+
 /* The reference test kernels over pre-encoded inputs (test/evaluate_NV.comp:16-31,
  * test/train_NV.comp:18-46): fp16 weights [weight_count], fp16 inputs [n][64].
  * forward: d_out [n][3] FP32 raw outputs.  gradient: L2 loss against d_targets

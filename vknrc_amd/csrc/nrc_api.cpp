@@ -563,6 +563,41 @@ nrc_status nrc_sph_encode(const float *d_dirs, uint32_t n, float *d_uv, void *st
 	return NRC_OK;
 }
 
+uint32_t nrc_dst_screen(uint32_t x, uint32_t y) { return ((x & 0x7FFFu) | ((y & 0x7FFFu) << 15)) << 1; }
+
+uint32_t nrc_dst_train(uint32_t b, uint32_t l, uint32_t r) {
+	return (((b & 3u) | ((l & 0x3FFFu) << 2) | ((r & 0x3FFFu) << 16)) << 1) | 1u;
+}
+
+nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32_t *d_count, uint32_t max_count,
+                         const nrc_screen *screen, nrc_train_record *d_batches, const float *d_train_factors,
+                         uint32_t batch_stride, void *stream) {
+	if (max_count == 0) return NRC_OK;
+	if (!d_rgb || !d_dst) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (screen && (!screen->d_bias_factor_r || !screen->d_factor_gb || screen->pitch < screen->width))
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "screen images NULL or pitch < width");
+	if (d_batches && (!d_train_factors || batch_stride == 0))
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "train factors NULL or batch_stride 0");
+	nrc::CompositeArgs a{};
+	a.rgb = d_rgb;
+	a.dst = d_dst;
+	a.d_count = d_count;
+	a.max_count = max_count;
+	if (screen) {
+		a.bias_factor_r = screen->d_bias_factor_r;
+		a.factor_gb = screen->d_factor_gb;
+		a.width = screen->width;
+		a.height = screen->height;
+		a.pitch = screen->pitch;
+	}
+	a.batches = reinterpret_cast<float *>(d_batches);
+	a.factors = d_train_factors;
+	a.batch_stride = batch_stride;
+	hipError_t e = nrc::launch_composite(a, reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_composite");
+	return NRC_OK;
+}
+
 nrc_status nrc_mlp_forward_f16(uint32_t hidden, const uint16_t *d_w, const uint16_t *d_in, uint32_t n, float *d_out,
                                void *stream) {
 	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");

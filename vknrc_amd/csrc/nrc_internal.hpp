@@ -135,6 +135,19 @@ hipError_t launch_plain_infer(int hidden, int encoding, const float *queries, co
 hipError_t launch_plain_gradient(const TrainArgs &a, uint32_t *nparts, hipStream_t stream);
 hipError_t launch_encode(int encoding, const float *queries, uint32_t n, float *features, hipStream_t stream);
 hipError_t launch_sph_encode(const float *dirs, uint32_t n, float *uv, hipStream_t stream);
+// The inference epilogue (screen composite and training-target back-fill).
+struct CompositeArgs {
+	const float *rgb;
+	const uint32_t *dst, *d_count;
+	uint32_t max_count;
+	float *bias_factor_r;
+	const float *factor_gb;
+	uint32_t width, height, pitch;
+	float *batches; // nrc_train_record rows of kTrainFloats floats
+	const float *factors;
+	uint32_t batch_stride;
+};
+hipError_t launch_composite(const CompositeArgs &a, hipStream_t stream);
 
 // nrc_optim.hip
 hipError_t launch_adam(const AdamArgs &a, hipStream_t stream);

@@ -372,4 +372,55 @@ hipError_t launch_sph_encode(const float *dirs, uint32_t n, float *uv, hipStream
 	return hipGetLastError();
 }
 
+namespace {
+// b + f * p with the product and the sum rounded separately: the empty asm makes
+// the product opaque, so the compiler cannot contract the pair into an FMA.
+__device__ __forceinline__ float mul_add_rn(float b, float f, float p) {
+	float m = f * p;
+	asm volatile("" : "+v"(m));
+	return b + m;
+}
+
+// nrc_inference.comp:50-72: one thread per query.
+__global__ __launch_bounds__(256) void composite_kernel(const CompositeArgs a) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	const uint32_t count = effective_count(a.d_count, a.max_count, a.max_count);
+	if (i >= count) return;
+	const uint32_t e = a.dst[i];
+	if (e == 0xFFFFFFFFu) return;
+	const float p0 = a.rgb[(size_t)i * 3], p1 = a.rgb[(size_t)i * 3 + 1], p2 = a.rgb[(size_t)i * 3 + 2];
+	const uint32_t v = e >> 1;
+	// fields decoded as DecodeNRCEvalDstScreen/Train do (NRCRecord.glsl:24-33): the
+	// top field keeps every remaining bit, so an over-wide y or r falls outside the
+	// image or the batch and is skipped
+	if ((e & 1u) == 0u) { // screen
+		const uint32_t x = v & 0x7FFFu, y = v >> 15;
+		if (a.bias_factor_r == nullptr || x >= a.width || y >= a.height) return;
+		const size_t px = (size_t)y * a.pitch + x;
+		float4 *bf = reinterpret_cast<float4 *>(a.bias_factor_r) + px;
+		const float4 b = *bf;
+		const float2 f = reinterpret_cast<const float2 *>(a.factor_gb)[px];
+		*bf = make_float4(mul_add_rn(b.x, b.w, p0), mul_add_rn(b.y, f.x, p1), mul_add_rn(b.z, f.y, p2), 0.0f);
+	} else { // train
+		const uint32_t b = v & 3u, l = (v >> 2) & 0x3FFFu, r = v >> 16;
+		if (a.batches == nullptr) return;
+		const uint32_t rr = r < a.batch_stride ? r : a.batch_stride - 1u;
+		for (uint32_t j = l; j <= rr && j < a.batch_stride; ++j) {
+			const size_t row = (size_t)b * a.batch_stride + j;
+			float *t = a.batches + row * kTrainFloats + kQueryFloats;
+			const float *f = a.factors + row * 3;
+			t[0] = mul_add_rn(t[0], f[0], p0);
+			t[1] = mul_add_rn(t[1], f[1], p1);
+			t[2] = mul_add_rn(t[2], f[2], p2);
+		}
+	}
+}
+} // namespace
+
+hipError_t launch_composite(const CompositeArgs &a, hipStream_t stream) {
+	if (a.max_count == 0) return hipSuccess;
+	composite_kernel<<<dim3((a.max_count + 255) / 256), dim3(256), 0, stream>>>(a);
+	return hipGetLastError();
+}
+
 } // namespace nrc

@@ -62,6 +62,12 @@ class OptimizerState(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Screen(C.Structure):
+    """nrc_screen: RGBA32F (bias rgb, factor r) and RG32F (factor gb) images [height][pitch]."""
+    _fields_ = [("d_bias_factor_r", C.c_void_p), ("d_factor_gb", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("pitch", C.c_uint32)]
+
+
 class GroupItem(C.Structure):
     """nrc_group_item: one cache's inputs and outputs in a grouped call."""
     _fields_ = [("cache", C.c_void_p), ("d_queries", C.c_void_p), ("d_query_count", C.c_void_p),
@@ -103,6 +109,11 @@ def load_library():
     L.nrc_get_fp8_scales.argtypes = [P, P, P, P]
     L.nrc_cache_footprint.argtypes = [C.POINTER(CacheDesc), C.c_int, P]
     L.nrc_cache_set_grad_chunks.argtypes = [P, U32]
+    L.nrc_composite.argtypes = [P, P, P, U32, C.POINTER(Screen), P, P, U32, P]
+    L.nrc_dst_screen.argtypes = [U32, U32]
+    L.nrc_dst_screen.restype = U32
+    L.nrc_dst_train.argtypes = [U32, U32, U32]
+    L.nrc_dst_train.restype = U32
     L.nrc_set_infer_workgroups_per_cu.argtypes = [U32]
     L.nrc_encode.argtypes = [C.c_int, P, U32, P, P]
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
@@ -340,6 +351,35 @@ class NRCState:
 # ---- stateless entry points
 def encode(queries, n: int, features, encoding: int = ENCODING_TRI12, stream=None) -> None:
     _check(load_library().nrc_encode(int(encoding), _ptr(queries), int(n), _ptr(features), _stream(stream)))
+
+
+DST_INVALID = 0xFFFFFFFF
+
+
+def dst_screen(x, y):
+    """Destination code of screen pixel (x, y) (EncodeNRCEvalDstScreen, NRCRecord.glsl:19)."""
+    return ((np.asarray(x, np.uint64) & 0x7FFF | ((np.asarray(y, np.uint64) & 0x7FFF) << 15)) << 1).astype(np.uint32)
+
+
+def dst_train(b, l, r):
+    """Destination code of training records l..r of batch b (EncodeNRCEvalDstTrain, NRCRecord.glsl:20-22)."""
+    v = (np.asarray(b, np.uint64) & 3) | ((np.asarray(l, np.uint64) & 0x3FFF) << 2) | \
+        ((np.asarray(r, np.uint64) & 0x3FFF) << 16)
+    return ((v << 1) | 1).astype(np.uint32)
+
+
+def composite(rgb, dst, max_count: int, count=None, bias_factor_r=None, factor_gb=None, width: int = 0,
+              height: int = 0, pitch: int | None = None, batches=None, train_factors=None, batch_stride: int = 0,
+              stream=None) -> None:
+    """The inference epilogue (nrc_composite): screen composite into bias_factor_r and back-fill of the
+    training targets of batches, from the inference output rgb and per-query destination codes."""
+    scr = None
+    if bias_factor_r is not None:
+        scr = Screen(bias_factor_r.data_ptr(), factor_gb.data_ptr(), int(width), int(height),
+                     int(pitch if pitch is not None else width))
+    _check(load_library().nrc_composite(_ptr(rgb), _ptr(dst), _ptr(count), int(max_count),
+                                        None if scr is None else C.byref(scr), _ptr(batches),
+                                        _ptr(train_factors), int(batch_stride), _stream(stream)))
 
 
 def sph_encode(dirs, n: int, uv, stream=None) -> None:

@@ -364,6 +364,28 @@ template <int LD> __device__ __forceinline__ void store_bcol_ld(_Float16 *dst, c
 		for (int j = 0; j < 8; ++j) dst[bfeat(c, g, j) * LD + col] = b[c][j];
 }
 
+// The same store as dword pairs: lanes 2m and 2m+1 hold the same 16 features of
+// records col and col+1, so one DPP swap of each half2 register with the
+// neighbour lane and one byte permute give the even lane feature f_2k of both
+// records and the odd lane feature f_2k+1 of both; each lane then writes one
+// dword where store_bcol_ld writes two halves (the LDS layout is the same).
+template <int LD> __device__ __forceinline__ void store_bcol_pairs(_Float16 *dst, const half8_t *b, int g, int col) {
+	const uint32_t odd = (uint32_t)col & 1u;
+	const uint32_t sel = odd ? 0x03020706u : 0x05040100u; // odd: (X.hi, V.hi); even: (V.lo, X.lo)
+#pragma unroll
+	for (int c = 0; c < 2; ++c) {
+		const uint4 v4 = __builtin_bit_cast(uint4, b[c]);
+		const uint32_t v[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			const uint32_t x = (uint32_t)__builtin_amdgcn_mov_dpp((int)v[k], 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+			const uint32_t w = __builtin_amdgcn_perm(x, v[k], sel);
+			const int f = bfeat(c, g, 2 * k) + (int)odd;
+			*reinterpret_cast<uint32_t *>(dst + f * LD + (col - (int)odd)) = w;
+		}
+	}
+}
+
 // reads back this lane's 16 values of a B operand (record column col) written by store_bcol_ld
 template <int LD> __device__ __forceinline__ void load_bcol_ld(const _Float16 *src, half8_t *b, int g, int col) {
 #pragma unroll
@@ -562,6 +584,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		// (the activations are not kept in registers for the backward: its ReLU masks
 		// come back from the LDS copy, which leaves room for the weight fragments)
 		half8_t B[2];
+		half8_t keep[H][2]; // a_1 .. a_{H-1} in B layout: the ReLU masks of the backward
 		load_bcol_ld<LD>(S.act[0], B, g, col);
 #pragma unroll
 		for (int l = 0; l < H; ++l) {
@@ -574,7 +597,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(Af[l][mt][c], B[c], acc[mt]);
 #pragma unroll
 			for (int c = 0; c < 2; ++c) B[c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
-			store_bcol_ld<LD>(S.act[l + 1], B, g, col);
+			if (l + 1 < H) {
+				keep[l + 1][0] = B[0];
+				keep[l + 1][1] = B[1];
+			}
+			store_bcol_pairs<LD>(S.act[l + 1], B, g, col);
 		}
 		float4_t o = (float4_t)0.0f;
 #pragma unroll
@@ -628,7 +655,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
 			relu_bwd_pack(acc, B, Bd); // B still holds a_H
-			store_bcol_ld<LD>(S.dlt[H - 1], Bd, g, col);
+			store_bcol_pairs<LD>(S.dlt[H - 1], Bd, g, col);
 		}
 #pragma unroll
 		for (int l = H - 1; l >= 1; --l) {
@@ -639,10 +666,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((l - 1) * 4 + mt) * 2 + c), Bd[c], acc[mt]);
-			half8_t al[2];
-			load_bcol_ld<LD>(S.act[l], al, g, col);
-			relu_bwd_pack(acc, al, Bd);
-			store_bcol_ld<LD>(S.dlt[l - 1], Bd, g, col);
+			relu_bwd_pack(acc, keep[l], Bd);
+			store_bcol_pairs<LD>(S.dlt[l - 1], Bd, g, col);
 		}
 		// this wave's loss sum, in a fixed order
 #pragma unroll

@@ -88,16 +88,16 @@ def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> d
 
 def group_workload(torch, v, wl, q_host, frames: int) -> dict:
     """16 caches: sequential nrc_frame calls vs one nrc_group_frame vs a replayed graph."""
-    rng = np.random.default_rng(wl.config_seed(3))
+    rng = np.random.default_rng(wl.workload_seed(wl.CACHES16))
     n = 16
     nq = np.exp(rng.uniform(np.log(65536), np.log(QUERIES), n)).astype(np.int64)
     nb = rng.choice([4096, 8192, 16384], n)
     dq = torch.from_numpy(q_host).to("cuda")
     entries = []
     for i in range(n):
-        st = v.NRCState(v.NRCConfig(seed=wl.config_seed(3, 100 + i), encoding=v.ENCODING_SINCOS6,
+        st = v.NRCState(v.NRCConfig(seed=wl.workload_seed(wl.CACHES16, 100 + i), encoding=v.ENCODING_SINCOS6,
                                     loss=v.LOSS_RELATIVE_L2_LUMINANCE, loss_scale=128.0, dynamic_loss_scale=True))
-        recs = wl.make_train_records(4 * int(nb[i]), rng, wl.GaussianLights(wl.config_seed(3, 200 + i)))
+        recs = wl.make_train_records(4 * int(nb[i]), rng, wl.GaussianLights(wl.workload_seed(wl.CACHES16, 200 + i)))
         entries.append(dict(cache=st, queries=dq[:int(nq[i])], max_queries=int(nq[i]),
                             rgb=torch.zeros((int(nq[i]), 3), dtype=torch.float32, device="cuda"),
                             batches=torch.from_numpy(recs).to("cuda"), max_batch_records=int(nb[i])))
@@ -143,8 +143,8 @@ def served_workload(torch, v, wl, dq, drec, sessions: int, budget_gib: float) ->
     .3/.4/.2/.1), 300 frames each of full-resolution inference + one 16,384-record training step,
     admitted by an HBM budget and run by continuous batching into grouped launches."""
     from vknrc_amd.serving import Server, poisson_sessions
-    rng = np.random.default_rng(wl.config_seed(4))
-    specs = poisson_sessions(sessions, 20.0, rng, frames=300, seed0=wl.config_seed(4, 1) % 100000)
+    rng = np.random.default_rng(wl.workload_seed(wl.SERVED))
+    specs = poisson_sessions(sessions, 20.0, rng, frames=300, seed0=wl.workload_seed(wl.SERVED, 1) % 100000)
     pool = torch.cat([dq] * 4)[:3840 * 2160]
 
     def make_config(spec):
@@ -185,13 +185,13 @@ def main() -> int:
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    cfg = v.NRCConfig(seed=wl.config_seed(1), encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE,
+    cfg = v.NRCConfig(seed=wl.workload_seed(wl.RADIANCE), encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE,
                       loss_scale=128.0, dynamic_loss_scale=True)
     st = v.NRCState(cfg)
-    rng = np.random.default_rng(wl.config_seed(1, 1))
+    rng = np.random.default_rng(wl.workload_seed(wl.RADIANCE, 1))
     log("generating synthetic data ...")
     q = wl.make_queries(QUERIES, rng)
-    lights = wl.GaussianLights(wl.config_seed(1, 2))
+    lights = wl.GaussianLights(wl.workload_seed(wl.RADIANCE, 2))
     pool_frames = 8
     recs = wl.make_train_records(pool_frames * BATCHES * BATCH, rng, lights)
     dq = torch.from_numpy(q).to(dev)

@@ -129,8 +129,8 @@ def test_fp8_frames_of_training_and_inference(o64, o32):
     every 50 frames against the FP8 oracle and the FP64 forward."""
     rng = np.random.default_rng(45)
     st = v.NRCState(v.NRCConfig(encoding=v.ENCODING_SINCOS6, loss_scale=128.0, dynamic_loss_scale=True,
-                                seed=wl.config_seed(2)))
-    lights = wl.GaussianLights(wl.config_seed(2, 2))
+                                seed=wl.workload_seed(wl.RADIANCE_FP8)))
+    lights = wl.GaussianLights(wl.workload_seed(wl.RADIANCE_FP8, 2))
     pool = 32
     recs = dev(wl.make_train_records(pool * 4 * 16384, rng, lights))
     qe = wl.make_queries(8192, rng)

@@ -48,9 +48,9 @@ def image_mse(pred, target):
 
 
 def test_image_mlp_learns_like_the_cpu_port_and_the_plain_path():
-    img = wl.make_image(wl.config_seed(0, 1))
+    img = wl.make_image(wl.workload_seed(wl.IMAGE_MLP, 1))
     kw = dict(hidden_layers=4, encoding=v.ENCODING_UV_SINCOS16, loss=v.LOSS_L2, init=v.INIT_XAVIER_UNIFORM,
-              seed=wl.config_seed(0), learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-8,
+              seed=wl.workload_seed(wl.IMAGE_MLP), learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-8,
               use_ema_weights=True)
     plain = v.NRCState(v.NRCConfig(**kw))
     fused = v.NRCState(v.NRCConfig(**kw))
@@ -63,7 +63,7 @@ def test_image_mlp_learns_like_the_cpu_port_and_the_plain_path():
         pass
     opt = OracleOptimizer(cpu, w0, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8)
 
-    rng = np.random.default_rng(wl.config_seed(0, 2))
+    rng = np.random.default_rng(wl.workload_seed(wl.IMAGE_MLP, 2))
     uv_eval = np.zeros((65536, wl.QUERY_FLOATS), np.float32)
     uv_eval[:, 0:2] = rng.uniform(0, 1, (65536, 2))
     target_eval = wl.sample_bilinear(img, uv_eval[:, 0:2].astype(np.float64)).astype(np.float64)
@@ -106,12 +106,12 @@ def test_image_mlp_learns_like_the_cpu_port_and_the_plain_path():
 
 def test_radiance_cache_loss_curve_fused_within_2pct_of_plain():
     kw = dict(encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE, loss_scale=128.0,
-              dynamic_loss_scale=True, seed=wl.config_seed(1))
+              dynamic_loss_scale=True, seed=wl.workload_seed(wl.RADIANCE))
     plain = v.NRCState(v.NRCConfig(**kw))
     fused = v.NRCState(v.NRCConfig(**kw))
-    rng = np.random.default_rng(wl.config_seed(1, 3))
+    rng = np.random.default_rng(wl.workload_seed(wl.RADIANCE, 3))
     pool = 128
-    recs = torch.from_numpy(wl.make_train_records(pool * BATCH, rng, wl.GaussianLights(wl.config_seed(1, 2)))).to(DEV)
+    recs = torch.from_numpy(wl.make_train_records(pool * BATCH, rng, wl.GaussianLights(wl.workload_seed(wl.RADIANCE, 2)))).to(DEV)
     lp, lf = [], []
     for s in range(STEPS):
         b = recs[(s % pool) * BATCH:][:BATCH]

@@ -24,8 +24,17 @@ TRAIN_FLOATS = 17
 MASTER_SEED = 0x4E52430000000000
 
 
-def config_seed(index: int, sub: int = 0) -> int:
-    return (MASTER_SEED + (index << 16) + sub) & 0xFFFFFFFFFFFFFFFF
+# the workloads whose seeds derive from MASTER_SEED
+IMAGE_MLP = 0      # 256x256 image, 64 -> 4x64 -> 3, UV_SINCOS16, L2 loss
+RADIANCE = 1       # one radiance cache, 2,073,600 queries + 4 x 16,384 training records a frame
+RADIANCE_FP8 = 2   # the same cache data, FP8 inference over 500 frames
+CACHES16 = 3       # 16 caches with uneven query and batch sizes
+SERVED = 4         # Poisson render sessions
+
+
+def workload_seed(workload: int, sub: int = 0) -> int:
+    """The seed of stream `sub` of a workload (one of the constants above)."""
+    return (MASTER_SEED + (workload << 16) + sub) & 0xFFFFFFFFFFFFFFFF
 
 
 def _sphere(rng, n):

@@ -394,15 +394,45 @@ template <int LD> __device__ __forceinline__ void load_bcol_ld(const _Float16 *s
 		for (int j = 0; j < 8; ++j) b[c][j] = src[bfeat(c, g, j) * LD + col];
 }
 
-// delta = relu_bwd(acc) against the activations in B-operand layout, packed to fp16
-__device__ __forceinline__ void relu_bwd_pack(const float4_t *acc, const half8_t *act, half8_t *d) {
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// The ReLU masks of a B operand of activations: all ones in each half whose fp16
+// activation is > 0.  0 - a saturates, so +0 and -0 (0x8000) both give 0 and
+// every positive value (finite or +Inf) a negative number; ReLU outputs are
+// never NaN (the pack's max returns its non-NaN operand).
+__device__ __forceinline__ uint4 act_masks(half8_t a) {
+	const uint4 w = __builtin_bit_cast(uint4, a);
+	auto m = [](uint32_t x) __attribute__((always_inline)) {
+		const short2_t t = __builtin_elementwise_sub_sat((short2_t)0, __builtin_bit_cast(short2_t, x));
+		return __builtin_bit_cast(uint32_t, (short2_t)(t >> 15));
+	};
+	return make_uint4(m(w.x), m(w.y), m(w.z), m(w.w));
+}
+
+// relu_bwd of two values packed to fp16: (fp16)x where the mask half is set and
+// the value is not NaN, else +0 (fp16 keeps NaN a NaN and takes values beyond
+// its range to +-Inf, so the NaN test is made on the converted halves: a half
+// is NaN when its magnitude bits exceed 0x7C00, i.e. when 0x7C00 - magnitude,
+// which stays within int16, is negative)
+__device__ __forceinline__ uint32_t relu_bwd2(float x, float y, uint32_t mask) {
+	const float2_t f = {x, y};
+	const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, half2_t));
+	short2_t mag = __builtin_bit_cast(short2_t, h & 0x7FFF7FFFu);
+	asm("" : "+v"(mag)); // keeps the test in packed 16-bit arithmetic
+	const short2_t nan = ((short2_t)0x7C00 - mag) >> 15; // all ones in a NaN half
+	return h & mask & ~__builtin_bit_cast(uint32_t, nan);
+}
+
+// delta = relu_bwd(acc) against the ReLU masks of the activations (B-operand layout), packed to fp16
+__device__ __forceinline__ void relu_bwd_pack(const float4_t *acc, const uint4 *mask, half8_t *d) {
 #pragma unroll
-	for (int c = 0; c < 2; ++c)
-#pragma unroll
-		for (int i = 0; i < 4; ++i) {
-			d[c][i] = (_Float16)relu_bwd(acc[2 * c][i], (float)act[c][i]);
-			d[c][4 + i] = (_Float16)relu_bwd(acc[2 * c + 1][i], (float)act[c][4 + i]);
-		}
+	for (int c = 0; c < 2; ++c) {
+		const uint4 o = make_uint4(relu_bwd2(acc[2 * c][0], acc[2 * c][1], mask[c].x),
+		                           relu_bwd2(acc[2 * c][2], acc[2 * c][3], mask[c].y),
+		                           relu_bwd2(acc[2 * c + 1][0], acc[2 * c + 1][1], mask[c].z),
+		                           relu_bwd2(acc[2 * c + 1][2], acc[2 * c + 1][3], mask[c].w));
+		d[c] = __builtin_bit_cast(half8_t, o);
+	}
 }
 
 template <int H, int ENC, int LOSS, bool F16IN, bool MULTI>
@@ -584,7 +614,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		// (the activations are not kept in registers for the backward: its ReLU masks
 		// come back from the LDS copy, which leaves room for the weight fragments)
 		half8_t B[2];
-		half8_t keep[H][2]; // a_1 .. a_{H-1} in B layout: the ReLU masks of the backward
+		uint4 keep[H][2]; // ReLU masks of a_1 .. a_{H-1} in B layout, for the backward
 		load_bcol_ld<LD>(S.act[0], B, g, col);
 #pragma unroll
 		for (int l = 0; l < H; ++l) {
@@ -598,8 +628,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 			for (int c = 0; c < 2; ++c) B[c] = pack_relu(acc[2 * c], acc[2 * c + 1]);
 			if (l + 1 < H) {
-				keep[l + 1][0] = B[0];
-				keep[l + 1][1] = B[1];
+				keep[l + 1][0] = act_masks(B[0]);
+				keep[l + 1][1] = act_masks(B[1]);
 			}
 			store_bcol_pairs<LD>(S.act[l + 1], B, g, col);
 		}
@@ -654,18 +684,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			float4_t acc[4];
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
-			relu_bwd_pack(acc, B, Bd); // B still holds a_H
+			const uint4 mH[2] = {act_masks(B[0]), act_masks(B[1])}; // B still holds a_H
+			relu_bwd_pack(acc, mH, Bd);
 			store_bcol_pairs<LD>(S.dlt[H - 1], Bd, g, col);
 		}
 #pragma unroll
 		for (int l = H - 1; l >= 1; --l) {
+			// the layer's eight W^T fragments are read from LDS together, so the
+			// MFMAs wait on one LDS round trip rather than on one each
+			half8_t wb[4][2];
+#pragma unroll
+			for (int c = 0; c < 2; ++c)
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt) wb[mt][c] = wt_frag(((l - 1) * 4 + mt) * 2 + c);
+			__builtin_amdgcn_sched_barrier(0);
 			float4_t acc[4];
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
 #pragma unroll
 			for (int c = 0; c < 2; ++c)
 #pragma unroll
-				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((l - 1) * 4 + mt) * 2 + c), Bd[c], acc[mt]);
+				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wb[mt][c], Bd[c], acc[mt]);
 			relu_bwd_pack(acc, keep[l], Bd);
 			store_bcol_pairs<LD>(S.dlt[l - 1], Bd, g, col);
 		}

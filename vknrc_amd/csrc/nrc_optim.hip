@@ -2,9 +2,12 @@
 // optimizer scalar recurrences (nrc_optimize.comp:32-54, nrc_train_prepare.comp:16-29).
 //
 // Each block owns 64 consecutive weights; its 256 threads sum the per-workgroup
-// partial gradients as float4 columns in 16 interleaved slices (sixteen loads in
+// partial gradients as float4 columns in 16 interleaved slices (U loads in
 // flight per thread) that are combined in a fixed order, so the reduced
-// gradient is bit-identical from run to run.  Weight counts are multiples of 64
+// gradient is bit-identical from run to run.  U is the smallest power of two
+// with 16 U >= the largest partial count of the launch (at most 16): with one
+// round of loads every U gives the sums of U = 16, since the accumulators a
+// smaller U leaves out would only add exact +0 to sums that start at +0.  Weight counts are multiples of 64
 // and partial rows are 64-float aligned (the row stride holds the loss column and
 // its 63 padding floats), so the float4 loads stay in bounds.  One extra block
 // reduces the loss column.  The scalar state (t, beta1^t, beta2^t, alpha^t,
@@ -17,7 +20,7 @@ namespace nrc {
 
 namespace {
 
-__global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
+template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	// this block's cache: the last item whose first block is at or below blockIdx.x
 	// (block_begin does not decrease, so k counts the later items starting at or below
 	// blockIdx.x; the unrolled count issues every table load at once)
@@ -45,22 +48,22 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	const uint32_t i = w0 + (uint32_t)t;
 	const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
 	const size_t ps4 = a.pstride / 4;
-	float4 acc[16];
+	float4 acc[U];
 #pragma unroll
-	for (int u = 0; u < 16; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	// 16 rows a round, all 16 loads in flight before the first add (rows past
+	for (int u = 0; u < U; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	// 16 U rows a round, all U loads in flight before the first add (rows past
 	// nparts load the last row and add as exact zeros, so the order of the sums is
 	// that of the rows)
 	const uint32_t lastrow = a.nparts - 1;
-	for (uint32_t q = slice; q < a.nparts; q += 256) {
-		float4 v[16];
+	for (uint32_t q = slice; q < a.nparts; q += 16 * U) {
+		float4 v[U];
 #pragma unroll
-		for (int u = 0; u < 16; ++u) {
+		for (int u = 0; u < U; ++u) {
 			const uint32_t row = q + 16 * u;
 			v[u] = src[(size_t)(row < lastrow ? row : lastrow) * ps4];
 		}
 #pragma unroll
-		for (int u = 0; u < 16; ++u) {
+		for (int u = 0; u < U; ++u) {
 			const bool in = q + 16 * u < a.nparts;
 			acc[u].x += in ? v[u].x : 0.0f;
 			acc[u].y += in ? v[u].y : 0.0f;
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	if (!skip) {
 		G4 = acc[0];
 #pragma unroll
-		for (int u = 1; u < 16; ++u) {
+		for (int u = 1; u < U; ++u) {
 			G4.x += acc[u].x; G4.y += acc[u].y; G4.z += acc[u].z; G4.w += acc[u].w;
 		}
 	}
@@ -203,13 +206,19 @@ __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 
 hipError_t launch_adam_group(AdamGroup &grp, hipStream_t stream) {
 	// per cache: one block per 64 weights plus one for the loss column
-	uint32_t blocks = 0;
+	uint32_t blocks = 0, maxparts = 0;
 	for (uint32_t i = 0; i < grp.n_items; ++i) {
 		grp.item[i].block_begin = blocks;
 		blocks += grp.item[i].weight_count / 64 + 1;
+		if (grp.item[i].nparts > maxparts) maxparts = grp.item[i].nparts;
 	}
 	if (blocks == 0) return hipSuccess;
-	adam_kernel<<<dim3(blocks), dim3(256), 0, stream>>>(grp);
+	const dim3 g(blocks), b(256);
+	if (maxparts <= 16) adam_kernel<1><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 32) adam_kernel<2><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 64) adam_kernel<4><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 128) adam_kernel<8><<<g, b, 0, stream>>>(grp);
+	else adam_kernel<16><<<g, b, 0, stream>>>(grp);
 	return hipGetLastError();
 }
 

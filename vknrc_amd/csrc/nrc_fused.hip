@@ -540,17 +540,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	for (int k = wave; k < H * 8; k += 4)
 		__builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(wtf + (k * 64 + lane) * 8),
 		                                 (__attribute__((address_space(3))) void *)(S.wt + k * 512), 16, 0, 0);
-	// the forward fragments stay in registers
+	// The forward fragments stay in registers.  A one-chunk workgroup fetches them
+	// once into the delta buffers (first written by the backward, which every wave
+	// reaches after its forward has consumed them) and each wave reads them from
+	// there after the first barrier, so the workgroup reads them from L2 once
+	// rather than once a wave; a multi-chunk workgroup, which holds them across its
+	// chunks, loads them straight into registers.
+	static_assert(sizeof(S.dlt) >= (H * 8 + 2) * 1024, "the fragment staging area is the delta buffers");
+	_Float16 *const stage = &S.dlt[0][0];
 	const half8_t *wfv = reinterpret_cast<const half8_t *>(wf);
 	half8_t Af[H][4][2], Ao[2];
+	auto fragments_from = [&](auto frag) __attribute__((always_inline)) {
 #pragma unroll
-	for (int l = 0; l < H; ++l)
+		for (int l = 0; l < H; ++l)
 #pragma unroll
-		for (int mt = 0; mt < 4; ++mt)
+			for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-			for (int c = 0; c < 2; ++c) Af[l][mt][c] = wfv[((l * 4 + mt) * 2 + c) * 64 + lane];
+				for (int c = 0; c < 2; ++c) Af[l][mt][c] = frag((l * 4 + mt) * 2 + c);
 #pragma unroll
-	for (int c = 0; c < 2; ++c) Ao[c] = wfv[(H * 8 + c) * 64 + lane];
+		for (int c = 0; c < 2; ++c) Ao[c] = frag(H * 8 + c);
+	};
+	if constexpr (MULTI) {
+		fragments_from([&](int k) { return wfv[k * 64 + lane]; });
+	} else {
+		for (int k = wave; k < H * 8 + 2; k += 4)
+			__builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(wf + (k * 64 + lane) * 8),
+			                                 (__attribute__((address_space(3))) void *)(stage + k * 512), 16, 0, 0);
+	}
 	auto wt_frag = [&](int blk) __attribute__((always_inline)) {
 		return *reinterpret_cast<const half8_t *>(S.wt + blk * 512 + lane * 8);
 	};
@@ -607,6 +623,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		}
 		if (ch == 0) __builtin_amdgcn_s_waitcnt(0); // this wave's fragment copies to LDS have landed
 		block_lds_sync();
+		if constexpr (!MULTI)
+			fragments_from([&](int k) { return *reinterpret_cast<const half8_t *>(stage + k * 512 + lane * 8); });
 		if constexpr (MULTI)
 			if (ch + 1 < nch) load_inputs(base + R); // the next chunk's inputs (uniform branch)
 

@@ -33,11 +33,11 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 	const int t = threadIdx.x;
 	DeviceState *st = a.state;
 	const int p = a.parity;
-	// The state, count, overflow flag and partial rows are all loaded before the
-	// first decision that depends on one, so a block waits on memory once for them:
-	// with few partial rows per cache the kernel is latency-bound.  (The optimizer
-	// entry is loaded where it is updated; moving that load changes which products
-	// the compiler contracts into FMAs in the update, and with them the last bit.)
+	// The state, count, overflow flag, partial rows and this thread's optimizer
+	// entry are all loaded before the first decision that depends on one, so a
+	// block waits on memory once for them: with few partial rows per cache the
+	// kernel is latency-bound.  (The update below spells out its fused
+	// multiply-adds, so where the entry is loaded does not change its rounding.)
 	const nrc_optimizer_state s0 = st->pub[p];
 	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
 	const uint32_t ovf = st->overflow[p];
@@ -46,6 +46,9 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 	const int col4 = t & 15, slice = t >> 4;
 	const uint32_t w0 = blk * 64u;
 	const uint32_t i = w0 + (uint32_t)t;
+	const bool updates = t < 64 && i < a.weight_count;
+	nrc_optimizer_entry e{};
+	if (updates) e = a.entries[i];
 	const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
 	const size_t ps4 = a.pstride / 4;
 	float4 acc[U];
@@ -71,7 +74,6 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 			acc[u].w += in ? v[u].w : 0.0f;
 		}
 	}
-	const bool updates = t < 64 && i < a.weight_count;
 
 	if (count == 0) { // nrc_optimize.comp:33-34: an empty batch changes nothing
 		if (blk == 0 && t == 0) {
@@ -113,13 +115,14 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 		for (int sl = 0; sl < 16; ++sl) G += rf[sl * 64 + t];
 		float g = G / (float)count / s0.loss_scale; // nrc_optimize.comp:36
 		if (!isfinite(g)) g = 0.0f;                 // nrc_optimize.comp:37-38
-		nrc_optimizer_entry e = a.entries[i];
-		e.m = b1 * e.m + (1.0f - b1) * g;
-		e.v = b2 * e.v + (1.0f - b2) * (g * g);
+		// the products that are fused into the sums are written as fmaf; every
+		// other product and quotient is rounded on its own
+		e.m = fmaf(1.0f - b1, g, b1 * e.m);
+		e.v = fmaf(1.0f - b2, g * g, b2 * e.v);
 		const float mh = e.m / (1.0f - B1), vh = e.v / (1.0f - B2);
 		e.weight -= a.p.lr * mh / (sqrtf(vh) + a.p.eps);
 		const float eta = 1.0f - A, eta1 = 1.0f - A1; // nrc_optimize.comp:46-47, as written
-		e.ema_weight = (1.0f - al) / eta * e.weight + al * eta1 * e.ema_weight;
+		e.ema_weight = fmaf(al * eta1, e.ema_weight, (1.0f - al) / eta * e.weight);
 		a.entries[i] = e;
 		a.w32[i] = e.weight;
 		const _Float16 h = (_Float16)e.weight;

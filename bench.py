@@ -14,8 +14,9 @@ The line also carries the 16-cache workload (`group16`): 16 independent caches
 with distinct seeded weights and light sets, per-frame query counts
 log-uniform in [65,536, 2,073,600] and training batches uniform in
 {4096, 8192, 16384}, timed as sequential per-cache frames, as one grouped frame
-and as a replayed HIP graph of the grouped frame; the FP8 inference (`fp8`);
-and the served workload (`served`): render sessions arriving as a Poisson
+and as a replayed HIP graph of the grouped frame; the FP8 inference (`fp8`); the
+inference epilogue over one 1920x1080 frame (`epilogue`: screen composite and
+training-target back-fill, with its HBM rate); and the served workload (`served`): render sessions arriving as a Poisson
 process of 20/s at 720p-2160p, 300 frames of inference + one training step
 each, admitted by an HBM budget and batched continuously into grouped launches
 (p50/p99 frame latency, admission waits, aggregate queries/s).
@@ -84,6 +85,44 @@ def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> d
             "sample": f"{reps} x {n} SINCOS6 queries inference (FP32, oracle/nrc_oracle.c REAL=float, OpenMP)",
             "train_samples_per_s": BATCH / t_tr,
             "train_sample": f"{treps} x {BATCH}-record forward+backward (no optimizer)"}
+
+
+def epilogue_workload(torch, v, rgb, drec, stream, reps: int) -> dict:
+    """nrc_composite over one 1920x1080 frame's inference output: the first 2,073,600 - 65,536 queries
+    composite into their pixels (in pixel order), the last 65,536 back-fill one training record each
+    (4 batches of 16,384)."""
+    W, H = 1920, 1080
+    n = rgb.shape[0]
+    T = BATCHES * BATCH
+    S = n - T
+    p = np.arange(S, dtype=np.int64)
+    j = np.arange(T, dtype=np.int64)
+    dst = np.concatenate([v.dst_screen(p % W, p // W), v.dst_train(j // BATCH, j % BATCH, j % BATCH)])
+    ddst = torch.from_numpy(dst).to("cuda")
+    g = torch.Generator(device="cuda").manual_seed(1)
+    bf = torch.rand((H, W, 4), device="cuda", generator=g)
+    fgb = torch.rand((H, W, 2), device="cuda", generator=g)
+    batches = drec[:T].clone()
+    factors = torch.rand((T, 3), device="cuda", generator=g)
+
+    def run():
+        v.composite(rgb, ddst, n, bias_factor_r=bf, factor_gb=fgb, width=W, height=H, batches=batches,
+                    train_factors=factors, batch_stride=BATCH, stream=stream)
+
+    for _ in range(3):
+        run()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    # every query reads its code (4 B) and radiance (12 B); a pixel reads and writes bias_factor_r (16 + 16 B)
+    # and reads factor_gb (8 B); a training record reads and writes its target (12 + 12 B) and reads its factor
+    nbytes = n * 16 + S * 40 + T * 36
+    return {"kernel": "composite_kernel", "ms_avg": ms, "queries": n, "screen_queries": S, "train_records": T,
+            "bytes": nbytes, "GB_s": nbytes / (ms / 1e3) / 1e9, "frac_hbm": nbytes / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
 
 
 def group_workload(torch, v, wl, q_host, frames: int) -> dict:
@@ -267,6 +306,8 @@ def main() -> int:
         plain_ms.append(e0.elapsed_time(e1))
     plain_ms = float(np.mean(plain_ms[1:]))
 
+    epilogue = epilogue_workload(torch, v, out, drec, stream, K)
+
     infer_avg_s = float(infer_ms.mean()) / 1e3
     achieved = INFER_FLOP_PER_QUERY * QUERIES / infer_avg_s / 1e12
     res = {
@@ -303,6 +344,7 @@ def main() -> int:
     res["fp8"] = {"infer_queries_per_s": QUERIES / f8_s, "infer_ms_avg": f8_s * 1e3,
                   "speedup_vs_f16": infer_avg_s / f8_s, "rel_err_vs_f16": f8_err,
                   "achieved_TFLOPs": f8_tflops, "peak_TFLOPs": PEAK_F8_TFLOPS, "frac": f8_tflops / PEAK_F8_TFLOPS}
+    res["epilogue"] = epilogue
     if args.group_frames > 0:
         log("16-cache workload ...")
         res["group16"] = group_workload(torch, v, wl, q, args.group_frames)

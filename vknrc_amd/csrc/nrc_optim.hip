@@ -20,6 +20,8 @@ namespace nrc {
 
 namespace {
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	// this block's cache: the last item whose first block is at or below blockIdx.x
 	// (block_begin does not decrease, so k counts the later items starting at or below
@@ -62,8 +64,10 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 		float4 v[U];
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
+			// non-temporal, as the gradient kernel stores the rows: nothing re-reads them
 			const uint32_t row = q + 16 * u;
-			v[u] = src[(size_t)(row < lastrow ? row : lastrow) * ps4];
+			const f4v x = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(src + (size_t)(row < lastrow ? row : lastrow) * ps4));
+			v[u] = make_float4(x[0], x[1], x[2], x[3]);
 		}
 #pragma unroll
 		for (int u = 0; u < U; ++u) {

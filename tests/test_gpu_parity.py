@@ -257,6 +257,24 @@ def test_fused_multi_chunk_train_step_matches_oracle(o64, n):
     assert s["state"]["last_count"] == n
 
 
+@pytest.mark.parametrize("n,chunks", [(64, 1), (1000, 1), (40000, 1), (40000, 2), (40000, 16)])
+def test_optimizer_reduction_over_any_partial_row_count(o64, n, chunks):
+    """1, 16, 625, 313 and 40 partial rows: the optimizer's one-round reductions (1 to 16 loads a slice)
+    and its multi-round loop (more than 256 rows) all give the oracle's gradient and loss."""
+    rng = np.random.default_rng(70 + chunks)
+    st, w = make_cache(rng, train_batch_size=65536, encoding=v.ENCODING_SINCOS6)
+    st.set_grad_chunks(chunks)
+    r = train_records(n, 71 + n % 5, v.ENCODING_SINCOS6)
+    st.train(dev(r), path=v.PATH_FUSED_F16)
+    s = st.get_state()
+    g = s["m"].astype(np.float64) / (1.0 - np.float32(0.9))
+    G, lsum = o64.gradient(f16(w), 5, r, v.ENCODING_SINCOS6, v.LOSS_RELATIVE_L2_LUMINANCE, round_f16=True)
+    assert cosine(g, G / n) > GRAD_COSINE_MIN, cosine(g, G / n)
+    assert rel_err(g, G / n) < 5e-2
+    assert abs(s["state"]["last_loss"] - lsum / n) < 1e-2 * abs(lsum / n)
+    assert s["state"]["last_count"] == n and s["state"]["t"] == 1
+
+
 def test_train_is_deterministic():
     rng = np.random.default_rng(5)
     w = rng.normal(0, 0.17, v.weight_count(5)).astype(np.float32)

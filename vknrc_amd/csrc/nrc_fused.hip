@@ -331,15 +331,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 //
 // One workgroup = 4 waves (one per SIMD) walking `chunks` 64-record chunks;
 // wave w runs records 16w .. 16w+15 of a chunk through the whole network.  The
-// forward A fragments (W_l) sit in registers, 168 for H = 5, loaded once with
-// 16-byte loads from the fragment-major copy; the backward ones (W_l^T) are
-// copied once per workgroup into LDS with direct global->LDS loads.  Waves write
-// their activations a_l and deltas feature-major into LDS as they produce them,
-// so the forward and the backward chain run without any barrier between waves.
-// After the one barrier, wave w computes its 2x2 block of dW_l^T tiles (input
-// tiles 2(w&1).., output tiles 2(w>>1)..) for every layer from LDS, K = the 64
-// records; a multi-chunk workgroup sums them in registers over its chunks, and
-// the tiles go (float4 per lane) to the workgroup's partial-gradient row.
+// forward A fragments (W_l) sit in registers, 168 for H = 5, loaded once from the
+// fragment-major copy (through LDS in a one-chunk workgroup); the backward ones
+// (W_l^T) are copied once per workgroup into LDS with direct global->LDS loads.
+// Waves write their activations a_l and deltas feature-major into LDS as they
+// produce them, so the forward and the backward chain run without any barrier
+// between waves; the backward takes its ReLU masks from registers.  After the
+// one barrier, wave w computes its 2x2 block of dW_l^T tiles (input tiles
+// 2(w&1).., output tiles 2(w>>1)..) for every layer from LDS, K = the 64 records;
+// a multi-chunk workgroup sums them in registers over its chunks, and the tiles
+// go (float4 per lane, non-temporal) to the workgroup's partial-gradient row.
 constexpr int kSmallLd = (int)kFusedSmallRecords + 8; // halves per LDS row
 
 template <int H> struct SmallGradSmem {

@@ -1,13 +1,15 @@
 // nrc_optim.hip -- partial-gradient reduction fused with Adam + EMA and the
 // optimizer scalar recurrences (nrc_optimize.comp:32-54, nrc_train_prepare.comp:16-29).
 //
-// Each block owns 64 consecutive weights; its 256 threads sum the per-workgroup
-// partial gradients as float4 columns in 16 interleaved slices (U loads in
-// flight per thread) that are combined in a fixed order, so the reduced
-// gradient is bit-identical from run to run.  U is the smallest power of two
-// with 16 U >= the largest partial count of the launch (at most 16): with one
-// round of loads every U gives the sums of U = 16, since the accumulators a
-// smaller U leaves out would only add exact +0 to sums that start at +0.  Weight counts are multiples of 64
+// Each block owns W groups of 64 consecutive weights; its 256 threads sum the
+// per-workgroup partial gradients as float4 columns in 16 interleaved slices
+// (U loads in flight per group and thread) that are combined in a fixed order,
+// so the reduced gradient is bit-identical from run to run.  U is the smallest
+// power of two with 16 U >= the largest partial count of the launch (at most
+// 16): with one round of loads every U gives the sums of U = 16, since the
+// accumulators a smaller U leaves out would only add exact +0 to sums that
+// start at +0.  W = 16 / U (at most 4) keeps 16 loads in flight a thread with
+// fewer, fuller blocks when the rows are few.  Weight counts are multiples of 64
 // and partial rows are 64-float aligned (the row stride holds the loss column and
 // its 63 padding floats), so the float4 loads stay in bounds.  One extra block
 // reduces the loss column.  The scalar state (t, beta1^t, beta2^t, alpha^t,
@@ -22,7 +24,9 @@ namespace {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
+// W: groups of 64 weights a block reduces (U W <= 16 loads a thread; each
+// weight's sums are the same whichever block reduces it).
+template <int U, int W> __global__ __launch_bounds__(256) void adam_kernel(const AdamGroup grp) {
 	// this block's cache: the last item whose first block is at or below blockIdx.x
 	// (block_begin does not decrease, so k counts the later items starting at or below
 	// blockIdx.x; the unrolled count issues every table load at once)
@@ -43,40 +47,53 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 	const nrc_optimizer_state s0 = st->pub[p];
 	const uint32_t count = effective_count(a.d_count, a.max_count, a.clamp);
 	const uint32_t ovf = st->overflow[p];
-	const bool loss_block = blk == a.weight_count / 64u;
-	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 4*col4 .. +3
+	const uint32_t wblocks = (a.weight_count + 64u * W - 1u) / (64u * W);
+	const bool loss_block = blk == wblocks;
+	// the block's columns: groups of 64 weights from w0 (the loss block: the loss
+	// column and its padding, one group)
+	const uint32_t w0 = loss_block ? a.weight_count : blk * 64u * W;
+	const uint32_t ngroups = loss_block ? 1u : ((a.weight_count - w0) / 64u < W ? (a.weight_count - w0) / 64u : W);
+	// thread t sums partials q = slice, slice + 16, ... of weights w0 + 64 gi + 4*col4 .. +3
 	const int col4 = t & 15, slice = t >> 4;
-	const uint32_t w0 = blk * 64u;
+	// and updates weight w0 + t
 	const uint32_t i = w0 + (uint32_t)t;
-	const bool updates = t < 64 && i < a.weight_count;
+	const bool updates = !loss_block && (uint32_t)t < 64u * ngroups;
 	nrc_optimizer_entry e{};
 	if (updates) e = a.entries[i];
 	const float4 *src = reinterpret_cast<const float4 *>(a.partials + w0) + col4;
 	const size_t ps4 = a.pstride / 4;
-	float4 acc[U];
+	float4 acc[W][U];
 #pragma unroll
-	for (int u = 0; u < U; ++u) acc[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	// 16 U rows a round, all U loads in flight before the first add (rows past
+	for (int gi = 0; gi < W; ++gi)
+#pragma unroll
+		for (int u = 0; u < U; ++u) acc[gi][u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	// 16 U rows a round, all U W loads in flight before the first add (rows past
 	// nparts load the last row and add as exact zeros, so the order of the sums is
-	// that of the rows)
+	// that of the rows; groups past the last weight load group 0 and are unused)
 	const uint32_t lastrow = a.nparts - 1;
 	for (uint32_t q = slice; q < a.nparts; q += 16 * U) {
-		float4 v[U];
+		float4 v[W][U];
 #pragma unroll
-		for (int u = 0; u < U; ++u) {
-			// non-temporal, as the gradient kernel stores the rows: nothing re-reads them
-			const uint32_t row = q + 16 * u;
-			const f4v x = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(src + (size_t)(row < lastrow ? row : lastrow) * ps4));
-			v[u] = make_float4(x[0], x[1], x[2], x[3]);
-		}
+		for (int gi = 0; gi < W; ++gi)
 #pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const bool in = q + 16 * u < a.nparts;
-			acc[u].x += in ? v[u].x : 0.0f;
-			acc[u].y += in ? v[u].y : 0.0f;
-			acc[u].z += in ? v[u].z : 0.0f;
-			acc[u].w += in ? v[u].w : 0.0f;
-		}
+			for (int u = 0; u < U; ++u) {
+				// non-temporal, as the gradient kernel stores the rows: nothing re-reads them
+				const uint32_t row = q + 16 * u;
+				const uint32_t col = (uint32_t)gi < ngroups ? 16u * gi : 0u;
+				const f4v x = __builtin_nontemporal_load(
+				    reinterpret_cast<const f4v *>(src + col + (size_t)(row < lastrow ? row : lastrow) * ps4));
+				v[gi][u] = make_float4(x[0], x[1], x[2], x[3]);
+			}
+#pragma unroll
+		for (int gi = 0; gi < W; ++gi)
+#pragma unroll
+			for (int u = 0; u < U; ++u) {
+				const bool in = q + 16 * u < a.nparts;
+				acc[gi][u].x += in ? v[gi][u].x : 0.0f;
+				acc[gi][u].y += in ? v[gi][u].y : 0.0f;
+				acc[gi][u].z += in ? v[gi][u].z : 0.0f;
+				acc[gi][u].w += in ? v[gi][u].w : 0.0f;
+			}
 	}
 
 	if (count == 0) { // nrc_optimize.comp:33-34: an empty batch changes nothing
@@ -87,21 +104,24 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 		return;
 	}
 	const bool skip = a.p.dynamic_scale && (ovf != 0);
-	__shared__ float4 red[16][16];
-	float4 G4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	if (!skip) {
-		G4 = acc[0];
+	__shared__ float4 red[W][16][16];
 #pragma unroll
-		for (int u = 1; u < U; ++u) {
-			G4.x += acc[u].x; G4.y += acc[u].y; G4.z += acc[u].z; G4.w += acc[u].w;
+	for (int gi = 0; gi < W; ++gi) {
+		float4 G4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		if (!skip) {
+			G4 = acc[gi][0];
+#pragma unroll
+			for (int u = 1; u < U; ++u) {
+				G4.x += acc[gi][u].x; G4.y += acc[gi][u].y; G4.z += acc[gi][u].z; G4.w += acc[gi][u].w;
+			}
 		}
+		red[gi][slice][col4] = G4;
 	}
-	red[slice][col4] = G4;
 	__syncthreads();
-	const float *rf = reinterpret_cast<const float *>(&red[0][0]);
 	if (loss_block) {
 		// column weight_count of the partials holds each workgroup's loss sum
 		if (t == 0) {
+			const float *rf = reinterpret_cast<const float *>(&red[0][0][0]);
 			float L = 0.0f;
 #pragma unroll
 			for (int sl = 0; sl < 16; ++sl) L += rf[sl * 64];
@@ -114,9 +134,10 @@ template <int U> __global__ __launch_bounds__(256) void adam_kernel(const AdamGr
 	const float B1 = s0.beta1_t * b1, B2 = s0.beta2_t * b2; // nrc_train_prepare.comp:23-27
 	const float A1 = s0.alpha_t, A = s0.alpha_t * al;
 	if (updates && !skip) {
+		const float *rf = reinterpret_cast<const float *>(&red[t >> 6][0][0]);
 		float G = 0.0f;
 #pragma unroll
-		for (int sl = 0; sl < 16; ++sl) G += rf[sl * 64 + t];
+		for (int sl = 0; sl < 16; ++sl) G += rf[sl * 64 + (t & 63)];
 		float g = G / (float)count / s0.loss_scale; // nrc_optimize.comp:36
 		if (!isfinite(g)) g = 0.0f;                 // nrc_optimize.comp:37-38
 		// the products that are fused into the sums are written as fmaf; every
@@ -212,20 +233,25 @@ __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 } // namespace
 
 hipError_t launch_adam_group(AdamGroup &grp, hipStream_t stream) {
-	// per cache: one block per 64 weights plus one for the loss column
-	uint32_t blocks = 0, maxparts = 0;
+	// per cache: one block per W groups of 64 weights plus one for the loss column;
+	// U loads a slice cover the largest partial count of the launch, and a block
+	// takes W = 16 / U groups (at most 4), so a thread has at most 16 loads in flight
+	uint32_t maxparts = 0;
+	for (uint32_t i = 0; i < grp.n_items; ++i)
+		if (grp.item[i].nparts > maxparts) maxparts = grp.item[i].nparts;
+	const uint32_t w = maxparts <= 64 ? 4u : maxparts <= 128 ? 2u : 1u;
+	uint32_t blocks = 0;
 	for (uint32_t i = 0; i < grp.n_items; ++i) {
 		grp.item[i].block_begin = blocks;
-		blocks += grp.item[i].weight_count / 64 + 1;
-		if (grp.item[i].nparts > maxparts) maxparts = grp.item[i].nparts;
+		blocks += (grp.item[i].weight_count + 64u * w - 1u) / (64u * w) + 1u;
 	}
 	if (blocks == 0) return hipSuccess;
 	const dim3 g(blocks), b(256);
-	if (maxparts <= 16) adam_kernel<1><<<g, b, 0, stream>>>(grp);
-	else if (maxparts <= 32) adam_kernel<2><<<g, b, 0, stream>>>(grp);
-	else if (maxparts <= 64) adam_kernel<4><<<g, b, 0, stream>>>(grp);
-	else if (maxparts <= 128) adam_kernel<8><<<g, b, 0, stream>>>(grp);
-	else adam_kernel<16><<<g, b, 0, stream>>>(grp);
+	if (maxparts <= 16) adam_kernel<1, 4><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 32) adam_kernel<2, 4><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 64) adam_kernel<4, 4><<<g, b, 0, stream>>>(grp);
+	else if (maxparts <= 128) adam_kernel<8, 2><<<g, b, 0, stream>>>(grp);
+	else adam_kernel<16, 1><<<g, b, 0, stream>>>(grp);
 	return hipGetLastError();
 }
 

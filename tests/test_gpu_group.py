@@ -123,7 +123,19 @@ def test_group_rejects_a_cache_twice():
     caches, entries = build(2, [1000], [1000], 31)
     with pytest.raises(v.NRCError):
         v.group_train([entries[0], entries[1], entries[0]], 0)
-    v.group_infer([entries[0], entries[0]])  # inference changes no state: allowed
+    # FP8 inference updates each cache's activation history and scales, so inference
+    # takes distinct caches too; a rejected frame launches nothing (its inference included)
+    for path in (v.PATH_FUSED_F16, v.PATH_FUSED_F8):
+        with pytest.raises(v.NRCError):
+            v.group_infer([entries[0], entries[0]], path=path)
+    out0 = entries[0]["rgb"].clone()
+    before = caches[0].get_state()
+    with pytest.raises(v.NRCError):
+        v.group_frame([entries[0], entries[1], entries[0]])
+    torch.cuda.synchronize()
+    assert torch.equal(entries[0]["rgb"], out0)
+    after = caches[0].get_state()
+    assert np.array_equal(before["weight"], after["weight"]) and after["state"]["t"] == before["state"]["t"]
 
 
 @pytest.mark.parametrize("chunks", [2, 8, 16])

@@ -310,3 +310,62 @@ def test_dynamic_loss_scale_policy(oracle64):
     assert opt.state.loss_scale == 128.0 and opt.state.t == 3
     # the scale divides out: g = G / count / s
     assert np.allclose(opt.m, 0.1 * 0.25 * (1 + 0.9 + 0.81))
+
+
+def test_adam_constant_gradient_moves_by_lr_every_step(oracle64):
+    """A gradient g held constant makes m_t = (1 - b1^t) g and v_t = (1 - b2^t) g^2, so the
+    bias-corrected moments are exactly g and g^2 and every step t = 1..20 moves each weight by
+    -lr g / (|g| + eps).  A missing or misplaced bias correction (m / (1 - b1) for instance, or a
+    correction with b^(t-1)) changes the step size from t = 2 on."""
+    rng = np.random.default_rng(41)
+    w0 = rng.normal(0, 1, 400)
+    g = rng.normal(0, 1, 400) * 10 ** rng.uniform(-7, 2, 400)
+    count, scale = 9, 16.0
+    opt = OracleOptimizer(oracle64, w0, loss_scale=scale)
+    step = 0.002 * g / (np.abs(g) + 1e-8)
+    for t in range(1, 21):
+        assert opt.step(g * count * scale, count)     # G = count * s * g
+        assert np.allclose(opt.m, (1 - 0.9 ** t) * g, rtol=1e-12, atol=0)
+        assert np.allclose(opt.v, (1 - 0.999 ** t) * g * g, rtol=1e-11, atol=0)
+        assert np.allclose(opt.w, w0 - t * step, rtol=1e-11, atol=1e-14), t
+
+
+def test_adam_sign_flip_closed_form(oracle64):
+    """g at step 1, -g at step 2: m_2 = 0.9 * 0.1 g - 0.1 g = -0.01 g, v_2 = 0.001 g^2 (0.999 + 1),
+    so the second step is -lr (-0.01 g / 0.19) / (|g| sqrt(1.999e-3 / 1.999e-3) + eps)."""
+    g = np.array([3.0, -0.5, 1e-3, 7e-9])
+    opt = OracleOptimizer(oracle64, np.zeros(4))
+    opt.step(g, 1)
+    w1 = opt.w.copy()
+    opt.step(-g, 1)
+    m2 = -0.01 * g
+    v2 = 0.001 * g * g * (0.999 + 1.0)
+    assert np.allclose(opt.m, m2, rtol=1e-13) and np.allclose(opt.v, v2, rtol=1e-13)
+    mh = m2 / (1 - 0.9 ** 2)
+    vh = v2 / (1 - 0.999 ** 2)                         # = g^2 exactly
+    assert np.allclose(vh, g * g, rtol=1e-12)
+    assert np.allclose(opt.w, w1 - 0.002 * mh / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18)
+
+
+def test_ema_as_written_with_frozen_weights(oracle64):
+    """lr = 0 keeps w fixed, so ema_t = c_t w with c_1 = 1 and, from nrc_optimize.comp:46-47 as
+    written (the second term is not divided by 1 - alpha^t),
+        c_2 = (1 - a) / (1 - a^2) + a (1 - a) = 1 / (1 + a) + a (1 - a),
+    which for a = 0.99 is 0.512412...; a bias-corrected EMA would keep c_t = 1.  As alpha^t -> 0
+    the recurrence becomes the plain EMA and c_t -> 1.  With a = 0 the EMA is the weight."""
+    w = np.array([1.0, -2.5, 0.125, 3e-4])
+    opt = OracleOptimizer(oracle64, w, lr=1e-300)       # a step of 1e-300 * O(1) leaves w as it is
+    opt.step(np.ones(4), 1)
+    assert np.array_equal(opt.w, w) and np.allclose(opt.ema, w, rtol=1e-15)       # c_1 = 1
+    opt.step(np.ones(4), 1)
+    a = 0.99
+    c2 = 1.0 / (1.0 + a) + a * (1.0 - a)
+    assert abs(c2 - 0.5124125628140703) < 1e-15
+    assert np.allclose(opt.ema, c2 * w, rtol=1e-12)
+    for _ in range(4000):                                # 0.99^4000 ~ 3.4e-18
+        opt.step(np.ones(4), 1)
+    assert np.allclose(opt.ema, w, rtol=1e-9)
+    z = OracleOptimizer(oracle64, w, lr=1e-300, ema_alpha=0.0)
+    z.step(np.ones(4), 1)
+    z.step(np.ones(4), 1)
+    assert np.array_equal(z.ema, w)

@@ -705,19 +705,37 @@ std::vector<std::vector<uint32_t>> group_launches(const nrc_group_item *items, c
 	return out;
 }
 
-static nrc_status group_train_impl(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
-                                   uint32_t flags, hipStream_t s) {
+// Argument checks of a grouped training step: distinct caches on the current
+// device, and every item's step (records, sizes, partial rows) valid.
+nrc_status check_group_train(const nrc_group_item *items, uint32_t n_items, nrc_path path) {
+	nrc_status st = check_group(items, n_items, true);
+	for (uint32_t i = 0; st == NRC_OK && i < n_items; ++i)
+		if (items[i].max_batch_records)
+			st = check_train(items[i].cache, items[i].d_batches, items[i].max_batch_records, path);
+	return st;
+}
+
+// Argument checks of a grouped inference: distinct caches (the FP8 path updates
+// each cache's activation history and scales) with their buffers.
+nrc_status check_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_path path) {
 	nrc_status st = check_group(items, n_items, true);
 	if (st != NRC_OK) return st;
+	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16 && path != NRC_PATH_FUSED_F8)
+		return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
+	for (uint32_t i = 0; i < n_items; ++i)
+		if (items[i].max_queries && (!items[i].d_queries || !items[i].d_rgb))
+			return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	return NRC_OK;
+}
+
+static nrc_status group_train_impl(const nrc_group_item *items, uint32_t n_items, uint32_t batch, nrc_path path,
+                                   uint32_t flags, hipStream_t s) {
 	if (batch >= NRC_TRAIN_BATCH_COUNT) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch >= NRC_TRAIN_BATCH_COUNT");
+	nrc_status st = check_group_train(items, n_items, path);
+	if (st != NRC_OK) return st;
 	std::vector<uint32_t> use;
-	for (uint32_t i = 0; i < n_items; ++i) {
-		const nrc_group_item &it = items[i];
-		if (it.max_batch_records == 0) continue;
-		st = check_train(it.cache, it.d_batches, it.max_batch_records, path);
-		if (st != NRC_OK) return st;
-		use.push_back(i);
-	}
+	for (uint32_t i = 0; i < n_items; ++i)
+		if (items[i].max_batch_records) use.push_back(i);
 	auto records = [&](const nrc_group_item &it) { return it.d_batches + (size_t)batch * it.max_batch_records; };
 	auto counts = [&](const nrc_group_item &it) { return it.d_batch_counts ? it.d_batch_counts + batch : nullptr; };
 	if (path == NRC_PATH_PLAIN_F32) {
@@ -755,16 +773,11 @@ static nrc_status group_train_impl(const nrc_group_item *items, uint32_t n_items
 extern "C" {
 
 nrc_status nrc_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
-	nrc_status st = check_group(items, n_items, false);
+	nrc_status st = check_group_infer(items, n_items, path);
 	if (st != NRC_OK) return st;
-	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16 && path != NRC_PATH_FUSED_F8)
-		return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
 	std::vector<uint32_t> use;
-	for (uint32_t i = 0; i < n_items; ++i) {
-		if (items[i].max_queries == 0) continue;
-		if (!items[i].d_queries || !items[i].d_rgb) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
-		use.push_back(i);
-	}
+	for (uint32_t i = 0; i < n_items; ++i)
+		if (items[i].max_queries) use.push_back(i);
 	if (path == NRC_PATH_PLAIN_F32) {
 		for (uint32_t i : use) {
 			const nrc_group_item &it = items[i];
@@ -802,7 +815,11 @@ nrc_status nrc_group_train(const nrc_group_item *items, uint32_t n_items, uint32
 }
 
 nrc_status nrc_group_frame(const nrc_group_item *items, uint32_t n_items, nrc_path path, void *stream) {
-	nrc_status st = nrc_group_infer(items, n_items, path, stream);
+	// every check before the first launch, so a rejected frame launches nothing
+	nrc_status st = check_group_infer(items, n_items, path);
+	if (st == NRC_OK) st = check_group_train(items, n_items, path);
+	if (st != NRC_OK) return st;
+	st = nrc_group_infer(items, n_items, path, stream);
 	for (uint32_t b = 0; st == NRC_OK && b < NRC_TRAIN_BATCH_COUNT; ++b)
 		st = group_train_impl(items, n_items, b, path, b == NRC_TRAIN_BATCH_COUNT - 1 ? NRC_TRAIN_WRITE_USE_WEIGHTS : 0u,
 		                      reinterpret_cast<hipStream_t>(stream));

@@ -798,23 +798,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // N-tiles each wave runs through the layers together (1 or 2; both measure the
 // same, and 1 needs no register spill); VKNRC_INFER_TILES overrides.
 int infer_tiles() {
-	static int t = 0;
-	if (t == 0) {
+	static const int t = [] {
 		const char *e = getenv("VKNRC_INFER_TILES");
-		t = (e && atoi(e) == 2) ? 2 : 1;
-	}
+		return (e && atoi(e) == 2) ? 2 : 1;
+	}();
 	return t;
 }
 
 
 // The software-pipelined inference kernel (VKNRC_INFER_PIPE=0 selects the other).
 bool infer_pipelined() {
-	static int p = -1;
-	if (p < 0) {
+	static const bool p = [] {
 		const char *e = getenv("VKNRC_INFER_PIPE");
-		p = (e && atoi(e) == 0) ? 0 : 1;
-	}
-	return p == 1;
+		return !(e && atoi(e) == 0);
+	}();
+	return p;
 }
 
 template <int H, int ENC, bool F16IN> hipError_t infer_launch(const InferGroup &grp, hipStream_t s) {
@@ -832,18 +830,19 @@ template <int H, int ENC, bool F16IN> hipError_t infer_launch(const InferGroup &
 	return hipGetLastError();
 }
 
-// The dynamic LDS size of both variants, set once (before any stream capture).
+// The dynamic LDS size of both variants, set once (before any stream capture; the
+// initialiser of a function-local static runs once even under concurrent first calls).
 template <int H, int ENC, int LOSS, bool F16IN> hipError_t grad_small_attributes() {
-	static hipError_t done = hipErrorNotReady;
-	if (done == hipErrorNotReady) {
+	static const hipError_t done = [] {
 		const int smem = (int)sizeof(SmallGradSmem<H>);
-		done = hipFuncSetAttribute(reinterpret_cast<const void *>(fused_grad_small_kernel<H, ENC, LOSS, F16IN, true>),
-		                           hipFuncAttributeMaxDynamicSharedMemorySize, smem);
-		if (done == hipSuccess)
-			done = hipFuncSetAttribute(
-			    reinterpret_cast<const void *>(fused_grad_small_kernel<H, ENC, LOSS, F16IN, false>),
-			    hipFuncAttributeMaxDynamicSharedMemorySize, smem);
-	}
+		hipError_t e = hipFuncSetAttribute(
+		    reinterpret_cast<const void *>(fused_grad_small_kernel<H, ENC, LOSS, F16IN, true>),
+		    hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+		if (e == hipSuccess)
+			e = hipFuncSetAttribute(reinterpret_cast<const void *>(fused_grad_small_kernel<H, ENC, LOSS, F16IN, false>),
+			                        hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+		return e;
+	}();
 	return done;
 }
 
@@ -904,16 +903,14 @@ uint32_t infer_wg_per_cu() {
 void set_infer_wg_per_cu(uint32_t v) { g_infer_wg_per_cu.store(v, std::memory_order_relaxed); }
 
 int device_cu_count() {
-	static int cus = 0;
-	if (cus == 0) {
+	static const int cus = [] {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		hipDeviceProp_t prop;
 		if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-			cus = prop.multiProcessorCount;
-		else
-			cus = 256;
-	}
+			return prop.multiProcessorCount;
+		return 256;
+	}();
 	return cus;
 }
 

@@ -53,38 +53,107 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> dict:
-    """The FP32 CPU port of the same MLP (oracle/, REAL=float), on a bounded sample."""
-    from oracle.oracle import Oracle
-    o = Oracle("f32")
+def cpu_model() -> str:
     try:
-        cores = min(16, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        cores = min(16, os.cpu_count() or 1)
-    o.set_threads(cores)
-    from vknrc_amd import nrc as vn
-    w = vn.init_weights(vn.NRCConfig(seed=1, encoding=vn.ENCODING_SINCOS6)).astype(np.float32)
-    n = 16384
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _rate(fn, items: int, budget_s: float) -> tuple[float, int]:
+    """items/s of fn() repeated for about budget_s seconds (at least twice), and the repetitions."""
+    fn()
     t0 = time.perf_counter()
     reps = 0
     while True:
-        o.infer(w, 5, queries[reps * n % (queries.shape[0] - n):][:n], 1, clamp0=True)
+        fn()
         reps += 1
-        if time.perf_counter() - t0 > budget_s * 0.6:
+        if time.perf_counter() - t0 > budget_s and reps >= 2:
             break
-    t_inf = (time.perf_counter() - t0) / reps
-    t1 = time.perf_counter()
-    treps = 0
-    while True:
-        o.gradient(w, 5, records[:BATCH], 1, 0)
-        treps += 1
-        if time.perf_counter() - t1 > budget_s * 0.4:
-            break
-    t_tr = (time.perf_counter() - t1) / treps
-    return {"value": n / t_inf, "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} x {n} SINCOS6 queries inference (FP32, oracle/nrc_oracle.c REAL=float, OpenMP)",
-            "train_samples_per_s": BATCH / t_tr,
-            "train_sample": f"{treps} x {BATCH}-record forward+backward (no optimizer)"}
+    return items * reps / (time.perf_counter() - t0), reps
+
+
+def cpu_baseline(queries: np.ndarray, records: np.ndarray, budget_s: float) -> dict:
+    """CPU baselines of the same MLP on the host cores, on bounded samples:
+    the Eigen MLP (oracle/eigen_mlp.cpp on the Eigen 3.4.0 the reference vendors; the reference test's
+    forward semantics, test/main.cpp:11-27) in FP32 on all cores and on one thread and in _Float16 on one
+    thread, its config-1 training and inference rates, and the FP32 C port of the whole path
+    (oracle/nrc_oracle.c REAL=float, encoding included)."""
+    from oracle import eigen
+    from oracle.oracle import ENC_SINCOS6, ENC_UV_SINCOS16, Oracle
+    from vknrc_amd import nrc as vn
+    from vknrc_amd import workloads as wl
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    port = Oracle("f32")
+    w = vn.init_weights(vn.NRCConfig(seed=1, encoding=vn.ENCODING_SINCOS6)).astype(np.float32)
+    n = 16384
+    q = queries[:n]
+    res = {"cores": cores, "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    share = budget_s / 7.0
+    if eigen.available():
+        mlp = eigen.EigenMLP()
+        x = port.encode(q, ENC_SINCOS6).astype(np.float32)
+        mlp.set_threads(cores)
+        r_all, k_all = _rate(lambda: mlp.forward_f32(w, 5, x), n, share)
+        mlp.set_threads(1)
+        r_one, k_one = _rate(lambda: mlp.forward_f32(w, 5, x), n, share)
+        n16 = 2048
+        w16, x16 = w.astype(np.float16), x[:n16].astype(np.float16)
+        r_16, k_16 = _rate(lambda: mlp.forward_f16(w16, 5, x16), n16, share)
+        # config 1: the image MLP (UV sin/cos 16 octaves, 64 -> 4 x 64 -> 3, L2, Adam 1e-2 / 0.9 / 0.99)
+        rng = np.random.default_rng(wl.workload_seed(wl.IMAGE_MLP, 9))
+        img = wl.make_image(wl.workload_seed(wl.IMAGE_MLP))
+        rec = wl.make_image_records(n, rng, img)
+        xi = port.encode(rec, ENC_UV_SINCOS16).astype(np.float32)
+        ti = np.ascontiguousarray(rec[:, 14:17])
+        wi = vn.init_weights(vn.NRCConfig(hidden_layers=4, encoding=vn.ENCODING_UV_SINCOS16,
+                                          init=vn.INIT_XAVIER_UNIFORM, seed=3)).astype(np.float32)
+        mi, vi = np.zeros_like(wi), np.zeros_like(wi)
+        mlp.set_threads(cores)
+        step = [0]
+
+        def train():
+            step[0] += 1
+            mlp.train_step_l2_f32(wi, mi, vi, 4, xi, ti, 1e-2, 0.9, 0.99, 1e-8, 0.9 ** step[0], 0.99 ** step[0])
+
+        r_tr, k_tr = _rate(train, n, share)
+        uv = wl.make_image_records(65536, rng, img)
+        xu = port.encode(uv, ENC_UV_SINCOS16).astype(np.float32)
+        r_inf1, k_inf1 = _rate(lambda: mlp.forward_f32(wi, 4, xu), 65536, share)
+        res.update({
+            "value": r_all, "unit": "queries/s", "kind": "port",
+            "sample": f"Eigen 3.4.0 FP32 forward of the reference test MLP (test/main.cpp:11-27 semantics), "
+                      f"{k_all} x {n} SINCOS6-encoded config-2 queries on {cores} threads (encoding excluded)",
+            "eigen_f32_1thread_queries_per_s": r_one,
+            "eigen_f16_1thread_queries_per_s": r_16,
+            "eigen_samples": f"1 thread: {k_one} x {n} FP32, {k_16} x {n16} _Float16 (every product and sum "
+                             f"rounded to FP16, as the reference test)",
+            "config1_train_samples_per_s": r_tr,
+            "config1_infer_queries_per_s": r_inf1,
+            "config1_samples": f"{k_tr} Adam steps of {n} image samples, {k_inf1} x 65,536 uv inferences, "
+                               f"{cores} threads, Eigen FP32 (encoding excluded)",
+        })
+    port.set_threads(cores)
+    p_all, pk_all = _rate(lambda: port.infer(w, 5, q, ENC_SINCOS6, clamp0=True), n, share)
+    port.set_threads(1)
+    p_one, pk_one = _rate(lambda: port.infer(w, 5, q[:4096], ENC_SINCOS6, clamp0=True), 4096, share)
+    port.set_threads(cores)
+    p_tr, pk_tr = _rate(lambda: port.gradient(w, 5, records[:BATCH], ENC_SINCOS6, 0), BATCH, share)
+    res.update({"port_queries_per_s": p_all, "port_1thread_queries_per_s": p_one,
+                "port_train_samples_per_s": p_tr,
+                "port_sample": f"FP32 C port (oracle/nrc_oracle.c REAL=float, OpenMP) with encoding: {pk_all} x {n} "
+                               f"queries on {cores} threads, {pk_one} x 4096 on 1 thread, {pk_tr} x {BATCH}-record "
+                               f"forward+backward on {cores} threads"})
+    if "value" not in res:   # no Eigen library on this host: the C port is the baseline
+        res.update({"value": p_all, "unit": "queries/s", "kind": "port", "sample": res["port_sample"]})
+    return res
 
 
 def epilogue_workload(torch, v, rgb, drec, stream, reps: int) -> dict:
@@ -127,19 +196,18 @@ def epilogue_workload(torch, v, rgb, drec, stream, reps: int) -> dict:
 
 def group_workload(torch, v, wl, q_host, frames: int) -> dict:
     """16 caches: sequential nrc_frame calls vs one nrc_group_frame vs a replayed graph."""
-    rng = np.random.default_rng(wl.workload_seed(wl.CACHES16))
-    n = 16
-    nq = np.exp(rng.uniform(np.log(65536), np.log(QUERIES), n)).astype(np.int64)
-    nb = rng.choice([4096, 8192, 16384], n)
+    spec = wl.caches16()
+    n = len(spec)
+    nq = np.array([c["queries"] for c in spec], np.int64)
+    nb = np.array([c["batch"] for c in spec], np.int64)
     dq = torch.from_numpy(q_host).to("cuda")
     entries = []
-    for i in range(n):
-        st = v.NRCState(v.NRCConfig(seed=wl.workload_seed(wl.CACHES16, 100 + i), encoding=v.ENCODING_SINCOS6,
-                                    loss=v.LOSS_RELATIVE_L2_LUMINANCE, loss_scale=128.0, dynamic_loss_scale=True))
-        recs = wl.make_train_records(4 * int(nb[i]), rng, wl.GaussianLights(wl.workload_seed(wl.CACHES16, 200 + i)))
-        entries.append(dict(cache=st, queries=dq[:int(nq[i])], max_queries=int(nq[i]),
-                            rgb=torch.zeros((int(nq[i]), 3), dtype=torch.float32, device="cuda"),
-                            batches=torch.from_numpy(recs).to("cuda"), max_batch_records=int(nb[i])))
+    for c in spec:
+        st = v.NRCState(v.NRCConfig(seed=c["seed"], encoding=v.ENCODING_SINCOS6, loss=v.LOSS_RELATIVE_L2_LUMINANCE,
+                                    loss_scale=128.0, dynamic_loss_scale=True))
+        entries.append(dict(cache=st, queries=dq[:c["queries"]], max_queries=c["queries"],
+                            rgb=torch.zeros((c["queries"], 3), dtype=torch.float32, device="cuda"),
+                            batches=torch.from_numpy(c["records"]).to("cuda"), max_batch_records=c["batch"]))
 
     def sequential():
         for e in entries:
@@ -203,7 +271,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain-reps", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=21.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-frames", type=int, default=30, help="timed frames of the 16-cache workload (0: skip)")
     ap.add_argument("--serve-sessions", type=int, default=60,

@@ -32,8 +32,11 @@ def weight_count(hidden: int) -> int:
 
 
 def build() -> None:
-    """Compile both oracle libraries (gcc, OpenMP)."""
+    """Compile both oracle libraries (gcc, OpenMP), and the Eigen CPU baseline where the
+    reference's vendored Eigen is present (oracle/eigen.py)."""
     subprocess.run(["make", "-s", "-C", HERE, "-B", "all"], check=True)
+    from oracle import eigen
+    eigen.build()
 
 
 class OptParams(C.Structure):

@@ -134,3 +134,20 @@ def make_image_records(n: int, rng: np.random.Generator, img: np.ndarray) -> np.
     r[:, 0:2] = uv
     r[:, QUERY_FLOATS:] = sample_bilinear(img, uv.astype(np.float64))
     return r
+
+
+def caches16(n: int = 16) -> list[dict]:
+    """The 16-cache workload: per cache, its query count (log-uniform in [65,536, 2,073,600]), its
+    training batch (uniform in {4096, 8192, 16384}), its weight seed and its 4 batches of training
+    records (Gaussian lights seeded per cache).  Every cache reads the first `queries` records of a
+    shared query buffer."""
+    rng = np.random.default_rng(workload_seed(CACHES16))
+    nq = np.exp(rng.uniform(np.log(65536), np.log(2073600), n)).astype(np.int64)
+    nb = rng.choice([4096, 8192, 16384], n)
+    out = []
+    for i in range(n):
+        recs = make_train_records(4 * int(nb[i]), rng, GaussianLights(workload_seed(CACHES16, 200 + i)))
+        out.append({"queries": int(nq[i]), "batch": int(nb[i]), "seed": workload_seed(CACHES16, 100 + i),
+                    "records": recs})
+    return out
+

@@ -266,6 +266,22 @@ nrc_status nrc_mlp_forward_f16(uint32_t hidden_layers, const uint16_t *d_weights
 nrc_status nrc_mlp_gradient_f16(uint32_t hidden_layers, const uint16_t *d_weights_f16, const uint16_t *d_inputs_f16,
                                 const float *d_targets, uint32_t n, float *d_dw, void *stream);
 
+/* Largest n nrc_mlp_gradient_f16_noalloc takes (its scratch is sized for it). */
+#define NRC_NOALLOC_MAX_GRADIENT_RECORDS 16384u
+
+/* nrc_mlp_gradient_f16 without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc).  Same kernels and results (bit for bit) as
+ * nrc_mlp_gradient_f16, with its scratch (the fragment-major weight copies and the
+ * partial-gradient rows) in a fixed device array of 21.3 MB that each GPU the
+ * library runs on holds until the process ends.  n <= NRC_NOALLOC_MAX_GRADIENT_RECORDS
+ * (NRC_ERROR_CAPACITY beyond).  All calls on one GPU share that array, so calls
+ * that overlap in time on one GPU (different streams or host threads without
+ * ordering between them) race; calls ordered on one stream do not. */
+nrc_status nrc_mlp_gradient_f16_noalloc(uint32_t hidden_layers, const uint16_t *d_weights_f16,
+                                        const uint16_t *d_inputs_f16, const float *d_targets, uint32_t n,
+                                        float *d_dw, void *stream);
+
 /*
  * Many caches at once.  The reference drives one cache per renderer
  * (NRCRenderGraph.cpp:46-80); these run the same per-cache passes for a group

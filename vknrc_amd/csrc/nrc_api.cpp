@@ -653,6 +653,83 @@ nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint
 
 } // extern "C"
 
+// ------------------------------------------------- gradient without allocation
+namespace {
+
+// Device scratch of nrc_mlp_gradient_f16_noalloc: the fragment-major weight
+// copies (forward, then backward) and the partial-gradient rows of a batch of up
+// to NRC_NOALLOC_MAX_GRADIENT_RECORDS records, one row per 64 records (the most
+// rows any gradient plan writes), for the largest network (5 hidden layers).
+constexpr size_t kNoallocWfBytes = 2 * (size_t)nrc::fwd_frag_halves(NRC_MAX_HIDDEN_LAYERS);
+constexpr size_t kNoallocWtfBytes = 2 * (size_t)nrc::bwd_frag_halves(NRC_MAX_HIDDEN_LAYERS);
+constexpr size_t kNoallocPstride = ((size_t)nrc::weight_count(NRC_MAX_HIDDEN_LAYERS) + 1 + 63) & ~size_t(63);
+constexpr size_t kNoallocRows = NRC_NOALLOC_MAX_GRADIENT_RECORDS / nrc::kFusedSmallRecords;
+constexpr size_t kNoallocBytes = kNoallocWfBytes + kNoallocWtfBytes + sizeof(float) * kNoallocPstride * kNoallocRows;
+static_assert(kNoallocWfBytes % 256 == 0 && kNoallocWtfBytes % 256 == 0, "scratch parts stay 256-byte aligned");
+
+// One copy per GPU, held until the process ends.  Host code never uses this name
+// as an address: the device address of each GPU's copy comes from
+// hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// The device address of this GPU's scratch, looked up once per device with the
+// device current (the first call on a device; later calls read the table).
+nrc_status noalloc_scratch(unsigned char **out) {
+	int dev = 0;
+	NRC_HIP(hipGetDevice(&dev), "hipGetDevice");
+	if (dev < 0 || dev >= kNoallocMaxDevices) return fail(NRC_ERROR_UNSUPPORTED, "device index beyond the scratch table");
+	std::lock_guard<std::mutex> lock(g_noalloc_mu);
+	if (!g_noalloc_ptr[dev]) {
+		void *p = nullptr;
+		NRC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+		if (!p) return fail(NRC_ERROR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+		g_noalloc_ptr[dev] = p;
+	}
+	*out = static_cast<unsigned char *>(g_noalloc_ptr[dev]);
+	return NRC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+nrc_status nrc_mlp_gradient_f16_noalloc(uint32_t hidden, const uint16_t *d_w, const uint16_t *d_in,
+                                        const float *d_targets, uint32_t n, float *d_dw, void *stream) {
+	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");
+	if (!d_w || !d_in || !d_targets || !d_dw) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (n > NRC_NOALLOC_MAX_GRADIENT_RECORDS)
+		return fail(NRC_ERROR_CAPACITY, "n > NRC_NOALLOC_MAX_GRADIENT_RECORDS (the scratch holds the partial rows of "
+		                                "at most 16384 records)");
+	unsigned char *base = nullptr;
+	nrc_status st = noalloc_scratch(&base);
+	if (st != NRC_OK) return st;
+	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+	const uint32_t wc = nrc_weight_count(hidden);
+	const uint32_t pstride = ((wc + 1) + 63u) & ~63u;
+	const size_t nf = nrc::fwd_frag_halves((int)hidden), nt = nrc::bwd_frag_halves((int)hidden);
+	_Float16 *wf = reinterpret_cast<_Float16 *>(base);
+	_Float16 *wtf = reinterpret_cast<_Float16 *>(base + kNoallocWfBytes);
+	float *partials = reinterpret_cast<float *>(base + kNoallocWfBytes + kNoallocWtfBytes);
+	// the fragment copies keep the last call's values: their padding must start at zero
+	hipError_t e = nrc::launch_zero_words(reinterpret_cast<uint32_t *>(wf), nf / 2, s);
+	if (e == hipSuccess) e = nrc::launch_zero_words(reinterpret_cast<uint32_t *>(wtf), nt / 2, s);
+	if (e == hipSuccess) e = nrc::launch_make_frags((int)hidden, reinterpret_cast<const _Float16 *>(d_w), wf, wtf, s);
+	uint32_t nparts = 0;
+	if (e == hipSuccess)
+		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, wf, wtf,
+		                                   partials, pstride, &nparts, s);
+	if (e == hipSuccess && (size_t)nparts > kNoallocRows) return fail(NRC_ERROR_CAPACITY, "gradient plan exceeds the scratch");
+	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
+	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16_noalloc");
+	return NRC_OK;
+}
+
+} // extern "C"
+
 // ------------------------------------------------------------------ groups
 struct nrc_graph {
 	hipGraph_t graph = nullptr;

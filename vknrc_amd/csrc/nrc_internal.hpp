@@ -159,6 +159,9 @@ hipError_t launch_init_state(DeviceState *st, float loss_scale, hipStream_t stre
 // Scatters row-major fp16 weights into the fragment-major copies (whose padding
 // must already be zero); wtf may be null.
 hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Float16 *wtf, hipStream_t stream);
+// Zeroes `words` 32-bit words at dst with a kernel (no memset node, so the launch
+// may be captured into a graph).
+hipError_t launch_zero_words(uint32_t *dst, size_t words, hipStream_t stream);
 
 // nrc_fused.hip
 // All items share (hidden, encoding); fills tile_begin and total_tiles.

@@ -212,6 +212,11 @@ __global__ __launch_bounds__(256) void make_frags_kernel(const _Float16 *__restr
 	frag_scatter(i, w16[i], wf, wtf);
 }
 
+__global__ __launch_bounds__(256) void zero_words_kernel(uint32_t *__restrict__ dst, size_t words) {
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = 0u;
+}
+
 __global__ void init_state_kernel(DeviceState *st, float loss_scale) {
 	nrc_optimizer_state s;
 	s.t = 0;
@@ -271,6 +276,14 @@ hipError_t launch_reduce_partials(const float *partials, uint32_t nparts, uint32
 hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Float16 *wtf, hipStream_t stream) {
 	const uint32_t n = (uint32_t)weight_count(hidden);
 	make_frags_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(w16, wf, wtf, n);
+	return hipGetLastError();
+}
+
+hipError_t launch_zero_words(uint32_t *dst, size_t words, hipStream_t stream) {
+	if (words == 0) return hipSuccess;
+	size_t blocks = (words + 255) / 256;
+	if (blocks > 1024) blocks = 1024;
+	zero_words_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, stream>>>(dst, words);
 	return hipGetLastError();
 }
 

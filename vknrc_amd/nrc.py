@@ -119,6 +119,7 @@ def load_library():
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
     L.nrc_mlp_gradient_f16.argtypes = [U32, P, P, P, U32, P, P]
+    L.nrc_mlp_gradient_f16_noalloc.argtypes = [U32, P, P, P, U32, P, P]
     GI = C.POINTER(GroupItem)
     L.nrc_group_infer.argtypes = [GI, U32, C.c_int, P]
     L.nrc_group_train.argtypes = [GI, U32, U32, C.c_int, U32, P]
@@ -129,7 +130,8 @@ def load_library():
     L.nrc_graph_destroy.restype = None
     for f in ("nrc_init_weights", "nrc_cache_create", "nrc_cache_reset", "nrc_cache_get_desc", "nrc_set_use_ema",
               "nrc_infer", "nrc_train", "nrc_frame", "nrc_get_state", "nrc_set_state", "nrc_encode",
-              "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16", "nrc_group_infer",
+              "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16", "nrc_mlp_gradient_f16_noalloc",
+              "nrc_group_infer",
               "nrc_group_train", "nrc_group_frame", "nrc_group_frame_graph", "nrc_graph_launch"):
         getattr(L, f).restype = C.c_int
     _lib = L
@@ -394,6 +396,13 @@ def mlp_forward_f16(weights_f16, inputs_f16, n: int, out, hidden_layers: int = 5
 def mlp_gradient_f16(weights_f16, inputs_f16, targets, n: int, dw, hidden_layers: int = 5, stream=None) -> None:
     _check(load_library().nrc_mlp_gradient_f16(int(hidden_layers), _ptr(weights_f16), _ptr(inputs_f16),
                                                _ptr(targets), int(n), _ptr(dw), _stream(stream)))
+
+
+def mlp_gradient_f16_noalloc(weights_f16, inputs_f16, targets, n: int, dw, hidden_layers: int = 5,
+                             stream=None) -> None:
+    """mlp_gradient_f16 with the library's fixed device scratch (no allocation); n <= 16384."""
+    _check(load_library().nrc_mlp_gradient_f16_noalloc(int(hidden_layers), _ptr(weights_f16), _ptr(inputs_f16),
+                                                       _ptr(targets), int(n), _ptr(dw), _stream(stream)))
 
 
 # ---- many caches at once

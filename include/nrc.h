@@ -257,6 +257,126 @@ nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32
                          const nrc_screen *screen, nrc_train_record *d_batches, const float *d_train_factors,
                          uint32_t batch_stride, void *stream);
 
+/* ---- record front-end: the renderer's wire records and the scene they point into.
+ * The reference's path tracer writes 16-byte packed path vertices (PackedNRCInput,
+ * NRCRecord.glsl:6-10; MakeNRCInput, path_tracer.comp:159-166) inside 20-byte eval
+ * records (NRCEvalRecord, NRCRecord.glsl:12-17) and 40-byte train records
+ * (NRCTrainRecord, NRCRecord.glsl:35-38); its shaders unpack them against the scene
+ * (UnpackNRCInput, NRCRecord.glsl:98-125; Scene.glsl:8-69). */
+typedef struct nrc_packed_input {
+	uint32_t primitive_id;
+	uint32_t flip_bit_instance_id; /* instance id | flip-normal bit << 31 */
+	uint32_t barycentric_2x16u;    /* unorm16 (b1, b2); b0 = 1 - b1 - b2 */
+	uint32_t scattered_dir_2x16u;  /* unorm16 sphere-encoded direction */
+} nrc_packed_input;
+
+typedef struct nrc_eval_record {
+	uint32_t dst; /* NRC_DST_INVALID, nrc_dst_screen(x, y) or nrc_dst_train(b, l, r) */
+	nrc_packed_input packed_input;
+} nrc_eval_record;
+
+/* bias: the radiance gathered after this vertex (the training target once the
+ * inference epilogue has added factor * the cache's prediction at the path's end) */
+typedef struct nrc_wire_train_record {
+	float bias[3], factor[3];
+	nrc_packed_input packed_input;
+} nrc_wire_train_record;
+
+/* Material (Scene.glsl:12-19, std430: 64 bytes); a texture id of 0xFFFFFFFF means
+ * the constant colour. */
+typedef struct nrc_material {
+	float diffuse[3];
+	uint32_t diffuse_texture_id;
+	float specular[3];
+	uint32_t specular_texture_id;
+	float emission[3];
+	uint32_t emission_texture_id;
+	float metallic, roughness, ior, pad;
+} nrc_material;
+
+/* An RGBA8 sRGB texture (VkScene.cpp:193), sampled with linear filtering and
+ * repeat addressing (NRCRenderGraph.cpp:132): each texel is decoded from sRGB
+ * before the bilinear blend. */
+typedef struct nrc_texture {
+	const uint32_t *d_rgba8; /* [height][width], R in the low byte */
+	uint32_t width, height;
+} nrc_texture;
+
+/* The scene buffers of Scene.glsl:26-47 (device pointers).  A vertex of primitive
+ * p is d_vertices[d_vertex_indices[3p + k]] transformed by its instance's
+ * d_transforms[i] (3 x 4 floats, row j giving world_j = row_j . (x, y, z, 1)). */
+typedef struct nrc_scene {
+	const float *d_vertices;            /* [vertices][3] */
+	const uint32_t *d_vertex_indices;   /* [primitives * 3] */
+	const float *d_texcoords;           /* [texcoords][2] */
+	const uint32_t *d_texcoord_indices; /* [primitives * 3] */
+	const nrc_material *d_materials;
+	const uint32_t *d_material_ids;     /* [primitives] */
+	const float *d_transforms;          /* [instances][3][4] */
+	const nrc_texture *d_textures;      /* device array of texture_count descriptors, or NULL */
+	uint32_t texture_count;
+} nrc_scene;
+
+/* UnpackNRCInput (NRCRecord.glsl:98-125) of the first min(*d_count, max_count)
+ * eval records (d_count NULL: max_count) into d_queries [max_count]; the records'
+ * dst codes stay in d_records for nrc_composite_records. */
+nrc_status nrc_unpack_eval_records(const nrc_scene *scene, const nrc_eval_record *d_records, const uint32_t *d_count,
+                                   uint32_t max_count, nrc_query *d_queries, void *stream);
+/* The same for train records, into the records nrc_train reads: the unpacked
+ * query and target = bias (nrc_gradient.comp:30-33). */
+nrc_status nrc_unpack_train_records(const nrc_scene *scene, const nrc_wire_train_record *d_records,
+                                    const uint32_t *d_count, uint32_t max_count, nrc_train_record *d_out,
+                                    void *stream);
+/* The inference epilogue over wire records (nrc_inference.comp:50-72): for each
+ * eval record i < count, with pred = d_rgb[i] (the output of nrc_infer on the
+ * unpacked queries): a screen code composites into the screen as nrc_composite;
+ * a train code (b, l, r) sets bias = bias + factor * pred in records l..r of
+ * d_batches + b * batch_stride (each product and sum rounded separately).  Records
+ * j >= batch_stride and pixels outside the screen are skipped. */
+nrc_status nrc_composite_records(const float *d_rgb, const nrc_eval_record *d_records, const uint32_t *d_count,
+                                 uint32_t max_count, const nrc_screen *screen, nrc_wire_train_record *d_batches,
+                                 uint32_t batch_stride, void *stream);
+
+/* Path vertices per path (path_tracer.comp:11, 256-257). */
+#define NRC_MAX_BOUNCE 8
+/* One traced path as ExtendedPathTrace (path_tracer.comp:250-372) holds it before
+ * its records are written: vertex i has its packed input, the light it sees
+ * (emission, or the constant sky light at a miss) and the colour (BRDF weight) of
+ * its scattering. */
+typedef struct nrc_traced_path {
+	uint32_t bounce;          /* vertices 0 .. bounce - 1 are valid, 1 <= bounce <= NRC_MAX_BOUNCE */
+	uint32_t batch;           /* train batch 0 .. NRC_TRAIN_BATCH_COUNT - 1 */
+	uint32_t light_terminate; /* the path ended at a light: no eval record for its end */
+	uint32_t pad;
+	float light[NRC_MAX_BOUNCE][3];
+	float color[NRC_MAX_BOUNCE][3];
+	nrc_packed_input input[NRC_MAX_BOUNCE];
+} nrc_traced_path;
+
+/* The tail of ExtendedPathTrace (path_tracer.comp:343-372) for n_paths paths:
+ * each takes `bounce` slots of its batch by an atomic add on
+ * d_batch_counts[batch]; if the first fits under batch_capacity, the suffix sums
+ * light[i] += color[i] * light[i + 1], color[i] *= color[i + 1] (i = bounce - 2 .. 0)
+ * give the bias and factor of its train records (those that fit), and, unless the
+ * path ended at a light, an eval record appended at d_eval_count with the train
+ * code (batch, first, last) and the packed input of vertex bounce - 1 asks the
+ * inference for the radiance past the path's end.  Record order within a batch
+ * follows the atomics; counts past the capacity keep growing, as the reference's
+ * do (consumers clamp them). */
+nrc_status nrc_build_train_records(const nrc_traced_path *d_paths, uint32_t n_paths, nrc_wire_train_record *d_batches,
+                                   uint32_t *d_batch_counts, uint32_t batch_capacity, nrc_eval_record *d_eval_records,
+                                   uint32_t *d_eval_count, uint32_t eval_capacity, void *stream);
+
+/* The screen pass (screen.frag:10-31) over an RGBA32F colour image [height][pitch]:
+ * when accumulate is set, colour = (colour + accumulated * count) / (count + 1)
+ * (just colour when count == 0) and the RGBA32F accumulation image receives
+ * (colour, 0); the output [height][pitch] RGBA32F pixel is
+ * (pow(filmic(colour), 1/2.2), 1) with the Hejl 2015 filmic curve at white
+ * point 3.2.  d_accumulate may be NULL when accumulate is 0. */
+nrc_status nrc_screen_tonemap(const float *d_color, float *d_accumulate, uint32_t width, uint32_t height,
+                              uint32_t pitch, int32_t accumulate, uint32_t accumulate_count, float *d_out,
+                              void *stream);
+
 /* The reference test kernels over pre-encoded inputs (test/evaluate_NV.comp:16-31,
  * test/train_NV.comp:18-46): fp16 weights [weight_count], fp16 inputs [n][64].
  * forward: d_out [n][3] FP32 raw outputs.  gradient: L2 loss against d_targets

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvknrc.so")
-SOURCES = ["nrc_api.cpp", "nrc_plain.hip", "nrc_fused.hip", "nrc_fp8.hip", "nrc_optim.hip"]
+SOURCES = ["nrc_api.cpp", "nrc_plain.hip", "nrc_fused.hip", "nrc_fp8.hip", "nrc_optim.hip", "nrc_records.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

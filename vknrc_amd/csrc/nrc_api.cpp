@@ -598,6 +598,90 @@ nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32
 	return NRC_OK;
 }
 
+} // extern "C"
+
+namespace {
+nrc_status check_scene(const nrc_scene *s) {
+	if (!s) return fail(NRC_ERROR_INVALID_ARGUMENT, "scene is NULL");
+	if (!s->d_vertices || !s->d_vertex_indices || !s->d_texcoords || !s->d_texcoord_indices || !s->d_materials ||
+	    !s->d_material_ids || !s->d_transforms)
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "scene buffer NULL");
+	if (s->texture_count && !s->d_textures) return fail(NRC_ERROR_INVALID_ARGUMENT, "texture table NULL");
+	return NRC_OK;
+}
+} // namespace
+
+extern "C" {
+
+nrc_status nrc_unpack_eval_records(const nrc_scene *scene, const nrc_eval_record *d_records, const uint32_t *d_count,
+                                   uint32_t max_count, nrc_query *d_queries, void *stream) {
+	if (max_count == 0) return NRC_OK;
+	nrc_status st = check_scene(scene);
+	if (st != NRC_OK) return st;
+	if (!d_records || !d_queries) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	hipError_t e = nrc::launch_unpack_eval(*scene, d_records, d_count, max_count, reinterpret_cast<float *>(d_queries),
+	                                       reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_unpack_eval_records");
+	return NRC_OK;
+}
+
+nrc_status nrc_unpack_train_records(const nrc_scene *scene, const nrc_wire_train_record *d_records,
+                                    const uint32_t *d_count, uint32_t max_count, nrc_train_record *d_out,
+                                    void *stream) {
+	if (max_count == 0) return NRC_OK;
+	nrc_status st = check_scene(scene);
+	if (st != NRC_OK) return st;
+	if (!d_records || !d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	hipError_t e = nrc::launch_unpack_train(*scene, d_records, d_count, max_count, reinterpret_cast<float *>(d_out),
+	                                        reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_unpack_train_records");
+	return NRC_OK;
+}
+
+nrc_status nrc_composite_records(const float *d_rgb, const nrc_eval_record *d_records, const uint32_t *d_count,
+                                 uint32_t max_count, const nrc_screen *screen, nrc_wire_train_record *d_batches,
+                                 uint32_t batch_stride, void *stream) {
+	if (max_count == 0) return NRC_OK;
+	if (!d_rgb || !d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (screen && (!screen->d_bias_factor_r || !screen->d_factor_gb || screen->pitch < screen->width))
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "screen images NULL or pitch < width");
+	if (d_batches && batch_stride == 0) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch_stride 0");
+	hipError_t e = nrc::launch_composite_records(
+	    d_rgb, d_records, d_count, max_count, screen ? screen->d_bias_factor_r : nullptr,
+	    screen ? screen->d_factor_gb : nullptr, screen ? screen->width : 0u, screen ? screen->height : 0u,
+	    screen ? screen->pitch : 0u, d_batches, batch_stride, reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_composite_records");
+	return NRC_OK;
+}
+
+nrc_status nrc_build_train_records(const nrc_traced_path *d_paths, uint32_t n_paths, nrc_wire_train_record *d_batches,
+                                   uint32_t *d_batch_counts, uint32_t batch_capacity, nrc_eval_record *d_eval_records,
+                                   uint32_t *d_eval_count, uint32_t eval_capacity, void *stream) {
+	if (n_paths == 0) return NRC_OK;
+	if (!d_paths || !d_batches || !d_batch_counts || !d_eval_records || !d_eval_count)
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	// the train code holds record indices in 14 bits (EncodeNRCEvalDstTrain, NRCRecord.glsl:20-22)
+	if (batch_capacity == 0 || batch_capacity > (1u << 14))
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "batch_capacity must be in [1, 16384]");
+	hipError_t e = nrc::launch_build_train_records(d_paths, n_paths, d_batches, d_batch_counts, batch_capacity,
+	                                               d_eval_records, d_eval_count, eval_capacity,
+	                                               reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_build_train_records");
+	return NRC_OK;
+}
+
+nrc_status nrc_screen_tonemap(const float *d_color, float *d_accumulate, uint32_t width, uint32_t height,
+                              uint32_t pitch, int32_t accumulate, uint32_t accumulate_count, float *d_out,
+                              void *stream) {
+	if (width == 0 || height == 0) return NRC_OK;
+	if (!d_color || !d_out || (accumulate && !d_accumulate)) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (pitch < width) return fail(NRC_ERROR_INVALID_ARGUMENT, "pitch < width");
+	hipError_t e = nrc::launch_screen(d_color, d_accumulate, width, height, pitch, accumulate ? 1 : 0, accumulate_count,
+	                                  d_out, reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_screen_tonemap");
+	return NRC_OK;
+}
+
 nrc_status nrc_mlp_forward_f16(uint32_t hidden, const uint16_t *d_w, const uint16_t *d_in, uint32_t n, float *d_out,
                                void *stream) {
 	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");

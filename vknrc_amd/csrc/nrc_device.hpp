@@ -222,6 +222,14 @@ template <int ENC> __device__ __forceinline__ void encode_all(const float *rec, 
 	}
 }
 
+// b + f * p with the product and the sum rounded separately: the empty asm makes
+// the product opaque, so the compiler cannot contract the pair into an FMA.
+__device__ __forceinline__ float mul_add_rn(float b, float f, float p) {
+	float m = f * p;
+	asm volatile("" : "+v"(m));
+	return b + m;
+}
+
 // NRCSphEncode, NRCRecord.glsl:47-49.
 __device__ __forceinline__ void sph_encode(float x, float y, float z, float *uv) {
 	constexpr float kPi = 3.14159265358979323846f;

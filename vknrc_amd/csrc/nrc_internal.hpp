@@ -163,6 +163,22 @@ hipError_t launch_make_frags(int hidden, const _Float16 *w16, _Float16 *wf, _Flo
 // may be captured into a graph).
 hipError_t launch_zero_words(uint32_t *dst, size_t words, hipStream_t stream);
 
+// nrc_records.hip: the record front-end, the wire-record epilogue, the path
+// suffix sums and the screen pass
+hipError_t launch_unpack_eval(const nrc_scene &s, const nrc_eval_record *rec, const uint32_t *d_count,
+                              uint32_t max_count, float *queries, hipStream_t stream);
+hipError_t launch_unpack_train(const nrc_scene &s, const nrc_wire_train_record *rec, const uint32_t *d_count,
+                               uint32_t max_count, float *out, hipStream_t stream);
+hipError_t launch_composite_records(const float *rgb, const nrc_eval_record *rec, const uint32_t *d_count,
+                                    uint32_t max_count, float *bias_factor_r, const float *factor_gb, uint32_t width,
+                                    uint32_t height, uint32_t pitch, nrc_wire_train_record *batches,
+                                    uint32_t batch_stride, hipStream_t stream);
+hipError_t launch_build_train_records(const nrc_traced_path *paths, uint32_t n_paths, nrc_wire_train_record *batches,
+                                      uint32_t *batch_counts, uint32_t capacity, nrc_eval_record *evals,
+                                      uint32_t *eval_count, uint32_t eval_capacity, hipStream_t stream);
+hipError_t launch_screen(const float *color, float *accumulate_img, uint32_t width, uint32_t height, uint32_t pitch,
+                         int accumulate, uint32_t count, float *out, hipStream_t stream);
+
 // nrc_fused.hip
 // All items share (hidden, encoding); fills tile_begin and total_tiles.
 hipError_t launch_fused_infer_group(int hidden, int encoding, InferGroup &grp, hipStream_t stream);

@@ -373,14 +373,6 @@ hipError_t launch_sph_encode(const float *dirs, uint32_t n, float *uv, hipStream
 }
 
 namespace {
-// b + f * p with the product and the sum rounded separately: the empty asm makes
-// the product opaque, so the compiler cannot contract the pair into an FMA.
-__device__ __forceinline__ float mul_add_rn(float b, float f, float p) {
-	float m = f * p;
-	asm volatile("" : "+v"(m));
-	return b + m;
-}
-
 // nrc_inference.comp:50-72: one thread per query.
 __global__ __launch_bounds__(256) void composite_kernel(const CompositeArgs a) {
 	const uint32_t i = blockIdx.x * 256u + threadIdx.x;

@@ -38,6 +38,23 @@ TRAIN_WRITE_USE_WEIGHTS = 1
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "HIP error",
           5: "capacity exceeded"}
 
+# Wire records of the renderer (include/nrc.h): numpy layouts of nrc_packed_input,
+# nrc_eval_record, nrc_wire_train_record, nrc_material and nrc_traced_path.
+MAX_BOUNCE = 8
+NO_TEXTURE = 0xFFFFFFFF
+PACKED_INPUT = np.dtype([("primitive_id", "<u4"), ("flip_bit_instance_id", "<u4"), ("barycentric_2x16u", "<u4"),
+                         ("scattered_dir_2x16u", "<u4")])
+EVAL_RECORD = np.dtype([("dst", "<u4"), ("packed_input", PACKED_INPUT)])
+WIRE_TRAIN_RECORD = np.dtype([("bias", "<f4", 3), ("factor", "<f4", 3), ("packed_input", PACKED_INPUT)])
+MATERIAL = np.dtype([("diffuse", "<f4", 3), ("diffuse_texture_id", "<u4"), ("specular", "<f4", 3),
+                     ("specular_texture_id", "<u4"), ("emission", "<f4", 3), ("emission_texture_id", "<u4"),
+                     ("metallic", "<f4"), ("roughness", "<f4"), ("ior", "<f4"), ("pad", "<f4")])
+TRACED_PATH = np.dtype([("bounce", "<u4"), ("batch", "<u4"), ("light_terminate", "<u4"), ("pad", "<u4"),
+                        ("light", "<f4", (MAX_BOUNCE, 3)), ("color", "<f4", (MAX_BOUNCE, 3)),
+                        ("input", PACKED_INPUT, MAX_BOUNCE)])
+assert (PACKED_INPUT.itemsize, EVAL_RECORD.itemsize, WIRE_TRAIN_RECORD.itemsize, MATERIAL.itemsize,
+        TRACED_PATH.itemsize) == (16, 20, 40, 64, 336)
+
 
 class NRCError(RuntimeError):
     def __init__(self, status: int, message: str):
@@ -66,6 +83,18 @@ class Screen(C.Structure):
     """nrc_screen: RGBA32F (bias rgb, factor r) and RG32F (factor gb) images [height][pitch]."""
     _fields_ = [("d_bias_factor_r", C.c_void_p), ("d_factor_gb", C.c_void_p), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("pitch", C.c_uint32)]
+
+
+class Texture(C.Structure):
+    """nrc_texture: an RGBA8 sRGB image on the device."""
+    _fields_ = [("d_rgba8", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Scene(C.Structure):
+    """nrc_scene: device pointers of the scene buffers (Scene.glsl:26-47)."""
+    _fields_ = [("d_vertices", C.c_void_p), ("d_vertex_indices", C.c_void_p), ("d_texcoords", C.c_void_p),
+                ("d_texcoord_indices", C.c_void_p), ("d_materials", C.c_void_p), ("d_material_ids", C.c_void_p),
+                ("d_transforms", C.c_void_p), ("d_textures", C.c_void_p), ("texture_count", C.c_uint32)]
 
 
 class GroupItem(C.Structure):
@@ -118,6 +147,12 @@ def load_library():
     L.nrc_encode.argtypes = [C.c_int, P, U32, P, P]
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
+    SC = C.POINTER(Scene)
+    L.nrc_unpack_eval_records.argtypes = [SC, P, P, U32, P, P]
+    L.nrc_unpack_train_records.argtypes = [SC, P, P, U32, P, P]
+    L.nrc_composite_records.argtypes = [P, P, P, U32, C.POINTER(Screen), P, U32, P]
+    L.nrc_build_train_records.argtypes = [P, U32, P, P, U32, P, P, U32, P]
+    L.nrc_screen_tonemap.argtypes = [P, P, U32, U32, U32, I32, U32, P, P]
     L.nrc_mlp_gradient_f16.argtypes = [U32, P, P, P, U32, P, P]
     L.nrc_mlp_gradient_f16_noalloc.argtypes = [U32, P, P, P, U32, P, P]
     GI = C.POINTER(GroupItem)
@@ -131,7 +166,8 @@ def load_library():
     for f in ("nrc_init_weights", "nrc_cache_create", "nrc_cache_reset", "nrc_cache_get_desc", "nrc_set_use_ema",
               "nrc_infer", "nrc_train", "nrc_frame", "nrc_get_state", "nrc_set_state", "nrc_encode",
               "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16", "nrc_mlp_gradient_f16_noalloc",
-              "nrc_group_infer",
+              "nrc_group_infer", "nrc_unpack_eval_records", "nrc_unpack_train_records", "nrc_composite_records",
+              "nrc_build_train_records", "nrc_screen_tonemap",
               "nrc_group_train", "nrc_group_frame", "nrc_group_frame_graph", "nrc_graph_launch"):
         getattr(L, f).restype = C.c_int
     _lib = L
@@ -382,6 +418,81 @@ def composite(rgb, dst, max_count: int, count=None, bias_factor_r=None, factor_g
     _check(load_library().nrc_composite(_ptr(rgb), _ptr(dst), _ptr(count), int(max_count),
                                         None if scr is None else C.byref(scr), _ptr(batches),
                                         _ptr(train_factors), int(batch_stride), _stream(stream)))
+
+
+class DeviceScene:
+    """A scene uploaded to the device (torch tensors) with its nrc_scene descriptor.
+
+    vertices [V][3] f32, vertex_indices [3P] u32, texcoords [T][2] f32, texcoord_indices [3P] u32,
+    materials: numpy array of MATERIAL, material_ids [P] u32, transforms [I][3][4] f32,
+    textures: list of [h][w] uint32 RGBA8 arrays (R in the low byte)."""
+
+    def __init__(self, torch, vertices, vertex_indices, texcoords, texcoord_indices, materials, material_ids,
+                 transforms, textures=()):
+        dev = "cuda"
+
+        def put(a, dt):
+            a = np.ascontiguousarray(np.asarray(a, dt))
+            return torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).to(dev)
+
+        self.vertices = put(vertices, np.float32)
+        self.vertex_indices = put(vertex_indices, np.uint32)
+        self.texcoords = put(texcoords, np.float32)
+        self.texcoord_indices = put(texcoord_indices, np.uint32)
+        mats = np.ascontiguousarray(np.asarray(materials, MATERIAL))
+        self.materials = torch.from_numpy(mats.view(np.uint8).copy()).to(dev)
+        self.material_ids = put(material_ids, np.uint32)
+        self.transforms = put(transforms, np.float32)
+        self.texture_images = [put(t, np.uint32) for t in textures]
+        desc = np.zeros(max(1, len(textures)), np.dtype([("p", "<u8"), ("w", "<u4"), ("h", "<u4")]))
+        for i, (img, host) in enumerate(zip(self.texture_images, textures)):
+            desc[i] = (img.data_ptr(), np.asarray(host).shape[1], np.asarray(host).shape[0])
+        self.texture_table = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+        self.c = Scene(self.vertices.data_ptr(), self.vertex_indices.data_ptr(), self.texcoords.data_ptr(),
+                       self.texcoord_indices.data_ptr(), self.materials.data_ptr(), self.material_ids.data_ptr(),
+                       self.transforms.data_ptr(), self.texture_table.data_ptr() if textures else None,
+                       len(textures))
+
+
+def unpack_eval_records(scene: DeviceScene, records, max_count: int, queries, count=None, stream=None) -> None:
+    """UnpackNRCInput of nrc_eval_record rows into nrc_query rows (nrc_unpack_eval_records)."""
+    _check(load_library().nrc_unpack_eval_records(C.byref(scene.c), _ptr(records), _ptr(count), int(max_count),
+                                                  _ptr(queries), _stream(stream)))
+
+
+def unpack_train_records(scene: DeviceScene, records, max_count: int, out, count=None, stream=None) -> None:
+    """nrc_wire_train_record rows -> nrc_train_record rows (query + target = bias)."""
+    _check(load_library().nrc_unpack_train_records(C.byref(scene.c), _ptr(records), _ptr(count), int(max_count),
+                                                   _ptr(out), _stream(stream)))
+
+
+def composite_records(rgb, records, max_count: int, count=None, bias_factor_r=None, factor_gb=None, width: int = 0,
+                      height: int = 0, pitch: int | None = None, batches=None, batch_stride: int = 0,
+                      stream=None) -> None:
+    """The inference epilogue over wire records (nrc_composite_records)."""
+    scr = None
+    if bias_factor_r is not None:
+        scr = Screen(bias_factor_r.data_ptr(), factor_gb.data_ptr(), int(width), int(height),
+                     int(pitch if pitch is not None else width))
+    _check(load_library().nrc_composite_records(_ptr(rgb), _ptr(records), _ptr(count), int(max_count),
+                                                None if scr is None else C.byref(scr), _ptr(batches),
+                                                int(batch_stride), _stream(stream)))
+
+
+def build_train_records(paths, n_paths: int, batches, batch_counts, batch_capacity: int, eval_records, eval_count,
+                        eval_capacity: int, stream=None) -> None:
+    """The path suffix sums and record appends of ExtendedPathTrace (nrc_build_train_records)."""
+    _check(load_library().nrc_build_train_records(_ptr(paths), int(n_paths), _ptr(batches), _ptr(batch_counts),
+                                                  int(batch_capacity), _ptr(eval_records), _ptr(eval_count),
+                                                  int(eval_capacity), _stream(stream)))
+
+
+def screen_tonemap(color, width: int, height: int, out, accumulate_img=None, accumulate: bool = False,
+                   accumulate_count: int = 0, pitch: int | None = None, stream=None) -> None:
+    """The screen pass (nrc_screen_tonemap): optional accumulation, filmic tone map and gamma."""
+    _check(load_library().nrc_screen_tonemap(_ptr(color), _ptr(accumulate_img), int(width), int(height),
+                                             int(pitch if pitch is not None else width), int(bool(accumulate)),
+                                             int(accumulate_count), _ptr(out), _stream(stream)))
 
 
 def sph_encode(dirs, n: int, uv, stream=None) -> None:

@@ -151,3 +151,88 @@ def caches16(n: int = 16) -> list[dict]:
                     "records": recs})
     return out
 
+
+
+SCENE = 5          # synthetic scenes and wire records for the record front-end
+
+
+def make_scene(seed: int, primitives: int = 4096, vertices: int = 3000, instances: int = 8, materials: int = 16,
+               textures: int = 4, texture_size: int = 64) -> dict:
+    """A random triangle scene in the layout of the reference's scene buffers (Scene.glsl:26-47):
+    shared vertices and texcoords indexed per primitive corner, instances with random rotations,
+    scales and translations (3x4 rows), materials of which about half sample one of the RGBA8 sRGB
+    textures for diffuse and a quarter for specular."""
+    import vknrc_amd.nrc as _n
+    rng = np.random.default_rng(seed)
+    s = {"vertices": rng.uniform(-1, 1, (vertices, 3)).astype(np.float32),
+         "vertex_indices": rng.integers(0, vertices, primitives * 3).astype(np.uint32),
+         "texcoords": rng.uniform(-0.5, 1.5, (vertices, 2)).astype(np.float32),
+         "texcoord_indices": rng.integers(0, vertices, primitives * 3).astype(np.uint32),
+         "material_ids": rng.integers(0, materials, primitives).astype(np.uint32)}
+    # degenerate triangles have no normal: redraw corners until every triangle has area
+    vi = s["vertex_indices"].reshape(-1, 3)
+    for _ in range(100):
+        v = s["vertices"][vi].astype(np.float64)
+        area = np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+        bad = area < 1e-3
+        if not bad.any():
+            break
+        vi[bad] = rng.integers(0, vertices, (int(bad.sum()), 3))
+    s["vertex_indices"] = vi.reshape(-1).astype(np.uint32)
+    T = np.zeros((instances, 3, 4), np.float32)
+    for i in range(instances):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        T[i, :, :3] = q * rng.uniform(0.5, 2.0)
+        T[i, :, 3] = rng.uniform(-3, 3, 3)
+    s["transforms"] = T
+    m = np.zeros(materials, _n.MATERIAL)
+    m["diffuse"] = rng.uniform(0, 1, (materials, 3))
+    m["specular"] = rng.uniform(0, 1, (materials, 3))
+    m["emission"] = 0.0
+    m["roughness"] = rng.uniform(0, 1, materials)
+    m["ior"] = 1.5
+    m["diffuse_texture_id"] = np.where(rng.uniform(size=materials) < 0.5,
+                                       rng.integers(0, max(textures, 1), materials), _n.NO_TEXTURE) if textures \
+        else _n.NO_TEXTURE
+    m["specular_texture_id"] = np.where(rng.uniform(size=materials) < 0.25,
+                                        rng.integers(0, max(textures, 1), materials), _n.NO_TEXTURE) if textures \
+        else _n.NO_TEXTURE
+    m["emission_texture_id"] = _n.NO_TEXTURE
+    s["materials"] = m
+    s["textures"] = [rng.integers(0, 2 ** 32, (texture_size, texture_size), dtype=np.uint64).astype(np.uint32)
+                     for _ in range(textures)]
+    return s
+
+
+def make_packed_inputs(scene: dict, n: int, rng: np.random.Generator) -> np.ndarray:
+    """n packed path vertices (MakeNRCInput, path_tracer.comp:159-166) on random primitives and
+    instances: unorm16 barycentrics inside the triangle and a unorm16 sphere-encoded direction."""
+    import vknrc_amd.nrc as _n
+    p = np.zeros(n, _n.PACKED_INPUT)
+    prims = len(scene["material_ids"])
+    p["primitive_id"] = rng.integers(0, prims, n)
+    flip = rng.integers(0, 2, n).astype(np.uint64)
+    p["flip_bit_instance_id"] = (rng.integers(0, len(scene["transforms"]), n).astype(np.uint64) | (flip << 31))
+    b = rng.uniform(0, 1, (n, 2))
+    out = b.sum(1) > 1
+    b[out] = 1 - b[out][:, ::-1]
+    bq = np.round(b * 65535).astype(np.uint32)
+    p["barycentric_2x16u"] = bq[:, 0] | (bq[:, 1] << 16)
+    d = sph_encode(_sphere(rng, n))
+    dq = np.round(np.clip(d, 0, 1) * 65535).astype(np.uint32)
+    p["scattered_dir_2x16u"] = dq[:, 0] | (dq[:, 1] << 16)
+    return p
+
+
+def make_traced_paths(scene: dict, n: int, rng: np.random.Generator) -> np.ndarray:
+    """n traced paths (nrc_traced_path): 1..8 vertices with random packed inputs, emissions and BRDF
+    colours, random train batches, a quarter ending at a light."""
+    import vknrc_amd.nrc as _n
+    P = np.zeros(n, _n.TRACED_PATH)
+    P["bounce"] = rng.integers(1, _n.MAX_BOUNCE + 1, n)
+    P["batch"] = rng.integers(0, 4, n)
+    P["light_terminate"] = rng.uniform(size=n) < 0.25
+    P["light"] = rng.lognormal(-1.0, 1.0, (n, _n.MAX_BOUNCE, 3))
+    P["color"] = rng.uniform(0, 1, (n, _n.MAX_BOUNCE, 3))
+    P["input"] = make_packed_inputs(scene, n * _n.MAX_BOUNCE, rng).reshape(n, _n.MAX_BOUNCE)
+    return P

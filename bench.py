@@ -194,6 +194,73 @@ def epilogue_workload(torch, v, rgb, drec, stream, reps: int) -> dict:
             "bytes": nbytes, "GB_s": nbytes / (ms / 1e3) / 1e9, "frac_hbm": nbytes / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
 
 
+def frontend_workload(torch, v, wl, reps: int) -> dict:
+    """The record front-end over one 1920x1080 frame's records, on a synthetic scene of 65,536 triangles
+    (8 instances, 32 materials, 8 RGBA8 sRGB textures of 256x256): 2,073,600 eval records unpacked
+    into queries, 4 x 16,384 train records unpacked into training records, the epilogue over the wire
+    records (2,008,064 screen codes in pixel order, 65,536 one-record back-fills) and the screen pass
+    with accumulation.  Each kernel is timed alone with HIP events; GB/s count the record bytes each
+    reads and writes (the scene gather, about 0.3 MB of triangles and 2 MB of textures, stays in L2)."""
+    n = v.nrc
+    W, H = 1920, 1080
+    NQ, T = QUERIES, BATCHES * BATCH
+    S = NQ - T
+    s = wl.make_scene(wl.workload_seed(wl.SCENE, 1), primitives=65536, vertices=40000, instances=8, materials=32,
+                      textures=8, texture_size=256)
+    ds = n.DeviceScene(torch, s["vertices"], s["vertex_indices"], s["texcoords"], s["texcoord_indices"],
+                       s["materials"], s["material_ids"], s["transforms"], s["textures"])
+    rng = np.random.default_rng(wl.workload_seed(wl.SCENE, 2))
+    rec = np.zeros(NQ, n.EVAL_RECORD)
+    p = np.arange(S)
+    rec["dst"][:S] = ((p % W) | ((p // W) << 15)) << 1
+    j = np.arange(T)
+    rec["dst"][S:] = ((((j // BATCH) | ((j % BATCH) << 2) | ((j % BATCH) << 16)) << 1) | 1).astype(np.uint32)
+    rec["packed_input"] = wl.make_packed_inputs(s, NQ, rng)
+    tr = np.zeros(T, n.WIRE_TRAIN_RECORD)
+    tr["bias"] = rng.uniform(0, 1, (T, 3))
+    tr["factor"] = rng.uniform(0, 1, (T, 3))
+    tr["packed_input"] = wl.make_packed_inputs(s, T, rng)
+    d_rec = torch.from_numpy(rec.view(np.uint8)).to("cuda")
+    d_tr = torch.from_numpy(tr.view(np.uint8)).to("cuda")
+    q = torch.zeros((NQ, 14), dtype=torch.float32, device="cuda")
+    t17 = torch.zeros((T, 17), dtype=torch.float32, device="cuda")
+    rgb = torch.rand((NQ, 3), device="cuda")
+    bf = torch.rand((H, W, 4), device="cuda")
+    fgb = torch.rand((H, W, 2), device="cuda")
+    acc = torch.zeros((H, W, 4), device="cuda")
+    out = torch.zeros((H, W, 4), device="cuda")
+    stream = torch.cuda.current_stream()
+    runs = {
+        "unpack_eval": (lambda: v.unpack_eval_records(ds, d_rec, NQ, q, stream=stream), NQ * (20 + 56)),
+        "unpack_train": (lambda: v.unpack_train_records(ds, d_tr, T, t17, stream=stream), T * (40 + 68)),
+        "composite_records": (lambda: v.composite_records(rgb, d_rec, NQ, bias_factor_r=bf, factor_gb=fgb, width=W,
+                                                          height=H, batches=d_tr, batch_stride=BATCH,
+                                                          stream=stream), NQ * 16 + S * 40 + T * 36),
+        "screen_accumulate_tonemap": (lambda: v.screen_tonemap(bf, W, H, out, accumulate_img=acc, accumulate=True,
+                                                               accumulate_count=3, stream=stream), W * H * 64),
+    }
+    # the same eval records sorted by primitive: the most coherent order a renderer could hand over
+    # (the order above, uniformly random primitives, is the least coherent)
+    order = np.argsort(rec["packed_input"]["primitive_id"], kind="stable")
+    d_rec_sorted = torch.from_numpy(rec[order].copy().view(np.uint8)).to("cuda")
+    runs["unpack_eval_sorted"] = (lambda: v.unpack_eval_records(ds, d_rec_sorted, NQ, q, stream=stream), NQ * (20 + 56))
+    res = {"scene": "65,536 triangles, 8 instances, 32 materials, 8 textures 256x256 RGBA8 sRGB; records on "
+                    "uniformly random primitives (unpack_eval_sorted: the same records sorted by primitive)"}
+    for name, (fn, nbytes) in runs.items():
+        for _ in range(3):
+            fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        res[name] = {"ms_avg": ms, "bytes": nbytes, "GB_s": nbytes / (ms / 1e3) / 1e9,
+                     "frac_hbm": nbytes / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
+    return res
+
+
 def group_workload(torch, v, wl, q_host, frames: int) -> dict:
     """16 caches: sequential nrc_frame calls vs one nrc_group_frame vs a replayed graph."""
     spec = wl.caches16()
@@ -249,7 +316,7 @@ def served_workload(torch, v, wl, dq, drec, sessions: int, budget_gib: float) ->
     """Render sessions with Poisson arrivals (mean 20/s) at 720p/1080p/1440p/2160p (weights
     .3/.4/.2/.1), 300 frames each of full-resolution inference + one 16,384-record training step,
     admitted by an HBM budget and run by continuous batching into grouped launches."""
-    from vknrc_amd.serving import Server, poisson_sessions
+    from vknrc_amd.serving import Server, grouped_vs_sequential, poisson_sessions
     rng = np.random.default_rng(wl.workload_seed(wl.SERVED))
     specs = poisson_sessions(sessions, 20.0, rng, frames=300, seed0=wl.workload_seed(wl.SERVED, 1) % 100000)
     pool = torch.cat([dq] * 4)[:3840 * 2160]
@@ -259,9 +326,15 @@ def served_workload(torch, v, wl, dq, drec, sessions: int, budget_gib: float) ->
                            loss_scale=128.0, dynamic_loss_scale=True)
 
     srv = Server(torch, pool, drec, make_config, budget_bytes=int(budget_gib * (1 << 30)), max_sessions=256)
-    rep = srv.run_realtime(specs)
+    # sessions are paced by a 60 frames/s clock: each lives 300 frames (5 s), so concurrency builds up
+    # towards 20/s x 5 s until the HBM budget or the 256-session cap holds it
+    rep = srv.run_realtime(specs, frame_interval=1.0 / 60.0)
     rep["arrival_rate_per_s"] = 20.0
     rep["hbm_budget_GiB"] = budget_gib
+    # one frame of as many sessions as ran at once at the peak (same resolution mix), grouped launches
+    # against per-session launches
+    k = max(1, min(int(rep["peak_sessions"]), len(specs)))
+    rep["vs_sequential"] = grouped_vs_sequential(torch, specs[:k], pool, drec, make_config)
     return rep
 
 
@@ -274,9 +347,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=21.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-frames", type=int, default=30, help="timed frames of the 16-cache workload (0: skip)")
-    ap.add_argument("--serve-sessions", type=int, default=60,
+    ap.add_argument("--serve-sessions", type=int, default=120,
                     help="sessions of the served workload (0: skip)")
-    ap.add_argument("--serve-budget-gib", type=float, default=48.0, help="HBM budget of the served workload")
+    ap.add_argument("--serve-budget-gib", type=float, default=24.0, help="HBM budget of the served workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run one elementwise copy (torch.neg) of 256 MiB fp32 -> 256 MiB, the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -405,14 +478,24 @@ def main() -> int:
         "roofline": {"bound": "mfma", "kernel": "fused_infer_pipe_kernel", "achieved": achieved,
                      "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_F16_TFLOPS,
                      "traffic": INFER_TRAFFIC_BYTES,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE then WRITE_SIZE of this kernel at this "
+                                       "workload, scaled by a 256 MiB copy kernel measured in the same runs "
+                                       "(profiles/fused_infer_2073600q_hbm_traffic.md); not read in this run",
                      "hbm_GBs": QUERY_BYTES * QUERIES / infer_avg_s / 1e9},
     }
+    train_sps = BATCHES * BATCH / (float(train_ms.mean()) / 1e3)
+    res["train_roofline"] = {"bound": "mfma", "kernels": "fused_grad_small_kernel + adam_kernel",
+                             "achieved": train_sps * TRAIN_FLOP_PER_SAMPLE / 1e12, "peak": PEAK_F16_TFLOPS,
+                             "unit": "TFLOP/s", "frac": train_sps * TRAIN_FLOP_PER_SAMPLE / 1e12 / PEAK_F16_TFLOPS,
+                             "step_us": float(train_ms.mean()) * 1e3 / BATCHES}
     f8_s = float(f8_ms.mean()) / 1e3
     f8_tflops = INFER_FLOP_PER_QUERY * QUERIES / f8_s / 1e12
     res["fp8"] = {"infer_queries_per_s": QUERIES / f8_s, "infer_ms_avg": f8_s * 1e3,
                   "speedup_vs_f16": infer_avg_s / f8_s, "rel_err_vs_f16": f8_err,
                   "achieved_TFLOPs": f8_tflops, "peak_TFLOPs": PEAK_F8_TFLOPS, "frac": f8_tflops / PEAK_F8_TFLOPS}
     res["epilogue"] = epilogue
+    log("record front-end ...")
+    res["frontend"] = frontend_workload(torch, v, wl, K)
     if args.group_frames > 0:
         log("16-cache workload ...")
         res["group16"] = group_workload(torch, v, wl, q, args.group_frames)

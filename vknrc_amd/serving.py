@@ -216,24 +216,42 @@ class Server:
             self._close(a)
         return len(ents)
 
-    def run_realtime(self, specs, max_in_flight: int = 2, time_limit: float = 600.0) -> dict:
-        """Arrivals on the wall clock (SessionSpec.arrival in seconds from the start)."""
+    def run_realtime(self, specs, max_in_flight: int = 2, time_limit: float = 600.0,
+                     frame_interval: float | None = None) -> dict:
+        """Arrivals on the wall clock (SessionSpec.arrival in seconds from the start).  With a
+        frame_interval (seconds, e.g. 1/60) the ticks follow a frame clock: a tick starts no earlier
+        than one interval after the previous one, as a renderer presenting at that rate would ask for
+        frames; a tick that starts later than its slot counts as late."""
         self.sched.submit(specs)
         t0 = time.perf_counter()
+        next_slot = 0.0
+        self.late_ticks = 0
         while not self.sched.idle():
             now = time.perf_counter() - t0
             if now > time_limit:
                 raise RuntimeError("serving run exceeded its time limit")
+            if frame_interval is not None and now < next_slot:
+                time.sleep(next_slot - now)
+                now = time.perf_counter() - t0
             n = self.tick(now)
             if n == 0:
                 nxt = self.sched.next_arrival()
                 if nxt is not None:
                     time.sleep(max(0.0, min(nxt - (time.perf_counter() - t0), 0.01)))
+                next_slot = time.perf_counter() - t0
                 continue
+            if frame_interval is not None:
+                if now > next_slot + frame_interval:
+                    self.late_ticks += 1
+                next_slot = max(next_slot + frame_interval, now)
             if len(self.ticks) > max_in_flight:  # keep the host at most a few ticks ahead of the device
                 self.ticks[-1 - max_in_flight][1].synchronize()
         self.torch.cuda.synchronize()
-        return self.report(time.perf_counter() - t0)
+        rep = self.report(time.perf_counter() - t0)
+        if frame_interval is not None:
+            rep["frame_interval_ms"] = frame_interval * 1e3
+            rep["late_ticks"] = self.late_ticks
+        return rep
 
     def run_ticks(self, specs) -> dict:
         """Arrivals on a tick clock (SessionSpec.arrival = the tick it arrives at)."""
@@ -268,3 +286,53 @@ class Server:
                 "queries": int(total_q), "wall_s": wall_s, "device_busy_s": busy_s,
                 "queries_per_s": total_q / wall_s if wall_s > 0 else 0.0,
                 "queries_per_device_s": total_q / busy_s if busy_s > 0 else 0.0}
+
+
+def grouped_vs_sequential(torch, specs, query_pool, record_pool, make_config, ticks: int = 5,
+                          path: int = _nrc.PATH_FUSED_F16) -> dict:
+    """Device time of one frame of every session in `specs` as the serving loop runs it (grouped
+    inference and training launches) against per-session launches (nrc_infer then nrc_train for
+    each session), averaged over `ticks` frames after one warm-up frame, each on its own caches."""
+    nb = record_pool.shape[0] // TRAIN_STEP_RECORDS
+
+    def sessions():
+        ents = []
+        for s in specs:
+            n = s.queries
+            off = (s.seed * 7919) % max(1, query_pool.shape[0] - n + 1)
+            b = s.seed % nb
+            ents.append(dict(cache=_nrc.NRCState(make_config(s)), queries=query_pool[off:off + n],
+                             rgb=torch.zeros((n, 3), dtype=torch.float32, device=query_pool.device),
+                             max_queries=n, batches=record_pool[b * TRAIN_STEP_RECORDS:(b + 1) * TRAIN_STEP_RECORDS],
+                             max_batch_records=TRAIN_STEP_RECORDS))
+        return ents
+
+    stream = torch.cuda.current_stream()
+
+    def grouped(ents):
+        _nrc.group_infer(ents, path=path, stream=stream)
+        _nrc.group_train(ents, 0, path=path, write_use_weights=True, stream=stream)
+
+    def sequential(ents):
+        for e in ents:
+            e["cache"].infer(e["queries"], e["rgb"], path=path, stream=stream)
+            e["cache"].train(e["batches"], path=path, write_use_weights=True, stream=stream)
+
+    out = {}
+    for name, fn in (("grouped", grouped), ("sequential", sequential)):
+        ents = sessions()
+        fn(ents)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(ticks):
+            fn(ents)
+        e1.record(stream)
+        e1.synchronize()
+        out[name] = e0.elapsed_time(e1) / ticks
+        for e in ents:
+            e["cache"].close()
+    return {"sessions": len(specs), "queries_per_frame": int(sum(s.queries for s in specs)),
+            "grouped_ms_per_frame": out["grouped"], "sequential_ms_per_frame": out["sequential"],
+            "grouped_vs_sequential": out["sequential"] / out["grouped"]}
+

@@ -35,26 +35,41 @@ __device__ __forceinline__ float srgb_decode(uint32_t c8) {
 	return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
 }
 
-__device__ __forceinline__ uint32_t wrap(int i, uint32_t n) {
-	const int m = i % (int)n;
-	return (uint32_t)(m < 0 ? m + (int)n : m);
+// The 256 sRGB decodes, built once per workgroup (one per thread of a 256-thread
+// block) so that a texel channel costs one LDS read instead of a powf.
+__device__ __forceinline__ void srgb_table_init(float *lut) {
+	lut[threadIdx.x] = srgb_decode(threadIdx.x);
+	__syncthreads();
+}
+
+// Texel coordinates of a repeat-addressed linear fetch along one axis of n texels:
+// the coordinate's fraction (in [0, 1]) scaled to texels, so the two texel indices
+// are in [-1, n] and wrap with one select each.  NaN or infinite coordinates give
+// texel 0 (v_cvt of NaN is 0), so every index stays inside the image.
+__device__ __forceinline__ void repeat_texels(float u, uint32_t n, uint32_t &i0, uint32_t &i1, float &a) {
+	const float uf = u - floorf(u);
+	const float x = uf * (float)n - 0.5f;
+	const float x0 = floorf(x);
+	a = x - x0;
+	const int k = (int)x0;
+	i0 = k < 0 ? n - 1u : (uint32_t)k < n ? (uint32_t)k : 0u;
+	i1 = (uint32_t)(k + 1) < n ? (uint32_t)(k + 1) : 0u;
 }
 
 // texture(sampler2D, uv).rgb with linear filtering and repeat addressing
 // (NRCRenderGraph.cpp:132) of an RGBA8 sRGB image (VkScene.cpp:193)
-__device__ void sample_texture(const nrc_texture &t, float u, float v, float *o) {
-	const float x = u * (float)t.width - 0.5f, y = v * (float)t.height - 0.5f;
-	const float fx0 = floorf(x), fy0 = floorf(y);
-	const float ax = x - fx0, ay = y - fy0;
-	const uint32_t x0 = wrap((int)fx0, t.width), x1 = wrap((int)fx0 + 1, t.width);
-	const uint32_t y0 = wrap((int)fy0, t.height), y1 = wrap((int)fy0 + 1, t.height);
+__device__ void sample_texture(const nrc_texture &t, float u, float v, const float *lut, float *o) {
+	uint32_t x0, x1, y0, y1;
+	float ax, ay;
+	repeat_texels(u, t.width, x0, x1, ax);
+	repeat_texels(v, t.height, y0, y1, ay);
 	const uint32_t p00 = t.d_rgba8[(size_t)y0 * t.width + x0], p01 = t.d_rgba8[(size_t)y0 * t.width + x1];
 	const uint32_t p10 = t.d_rgba8[(size_t)y1 * t.width + x0], p11 = t.d_rgba8[(size_t)y1 * t.width + x1];
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		const uint32_t sh = 8u * (uint32_t)c;
-		const float c00 = srgb_decode((p00 >> sh) & 255u), c01 = srgb_decode((p01 >> sh) & 255u);
-		const float c10 = srgb_decode((p10 >> sh) & 255u), c11 = srgb_decode((p11 >> sh) & 255u);
+		const float c00 = lut[(p00 >> sh) & 255u], c01 = lut[(p01 >> sh) & 255u];
+		const float c10 = lut[(p10 >> sh) & 255u], c11 = lut[(p11 >> sh) & 255u];
 		const float top = c00 + ax * (c01 - c00), bot = c10 + ax * (c11 - c10);
 		o[c] = top + ay * (bot - top);
 	}
@@ -63,7 +78,7 @@ __device__ void sample_texture(const nrc_texture &t, float u, float v, float *o)
 // GetSceneDiffuse / GetSceneSpecular (Scene.glsl:63-68); an id past the texture
 // table, or an empty texture, takes the constant colour
 __device__ __forceinline__ void material_color(const nrc_scene &s, const float *constant, uint32_t tex, float u, float v,
-                                               float *o) {
+                                               const float *lut, float *o) {
 	const nrc_texture *t = (tex == kNoTexture || s.d_textures == nullptr || tex >= s.texture_count) ? nullptr
 	                                                                                               : s.d_textures + tex;
 	if (t == nullptr || t->width == 0u || t->height == 0u || t->d_rgba8 == nullptr) {
@@ -71,12 +86,12 @@ __device__ __forceinline__ void material_color(const nrc_scene &s, const float *
 		o[1] = constant[1];
 		o[2] = constant[2];
 	} else {
-		sample_texture(*t, u, v, o);
+		sample_texture(*t, u, v, lut, o);
 	}
 }
 
 // UnpackNRCInput (NRCRecord.glsl:98-125) into the 14 floats of an nrc_query
-__device__ void unpack_input(const nrc_scene &s, const nrc_packed_input &p, float *q) {
+__device__ void unpack_input(const nrc_scene &s, const nrc_packed_input &p, const float *lut, float *q) {
 	const uint32_t prim = p.primitive_id, inst = p.flip_bit_instance_id & 0x7FFFFFFFu;
 	const bool flip = (p.flip_bit_instance_id >> 31) != 0u;
 	float v0[3], v1[3], v2[3];
@@ -106,8 +121,8 @@ __device__ void unpack_input(const nrc_scene &s, const nrc_packed_input &p, floa
 	                s.d_texcoords[(size_t)t2 * 2] * b[2];
 	const float v = s.d_texcoords[(size_t)t0 * 2 + 1] * b[0] + s.d_texcoords[(size_t)t1 * 2 + 1] * b[1] +
 	                s.d_texcoords[(size_t)t2 * 2 + 1] * b[2];
-	material_color(s, m.diffuse, m.diffuse_texture_id, u, v, q + 8);
-	material_color(s, m.specular, m.specular_texture_id, u, v, q + 11);
+	material_color(s, m.diffuse, m.diffuse_texture_id, u, v, lut, q + 8);
+	material_color(s, m.specular, m.specular_texture_id, u, v, lut, q + 11);
 }
 
 __device__ __forceinline__ nrc_packed_input load_packed(const uint32_t *w) {
@@ -122,11 +137,13 @@ __device__ __forceinline__ nrc_packed_input load_packed(const uint32_t *w) {
 __global__ __launch_bounds__(256) void unpack_eval_kernel(const nrc_scene s, const nrc_eval_record *__restrict__ rec,
                                                           const uint32_t *d_count, uint32_t max_count,
                                                           float *__restrict__ out) {
+	__shared__ float lut[256];
+	srgb_table_init(lut);
 	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	if (i >= effective_count(d_count, max_count, max_count)) return;
 	const uint32_t *w = reinterpret_cast<const uint32_t *>(rec + i);
 	float q[kQueryFloats];
-	unpack_input(s, load_packed(w + 1), q);
+	unpack_input(s, load_packed(w + 1), lut, q);
 	float2 *o = reinterpret_cast<float2 *>(out + (size_t)i * kQueryFloats);
 #pragma unroll
 	for (int k = 0; k < kQueryFloats / 2; ++k) o[k] = make_float2(q[2 * k], q[2 * k + 1]);
@@ -136,11 +153,13 @@ __global__ __launch_bounds__(256) void unpack_train_kernel(const nrc_scene s,
                                                            const nrc_wire_train_record *__restrict__ rec,
                                                            const uint32_t *d_count, uint32_t max_count,
                                                            float *__restrict__ out) {
+	__shared__ float lut[256];
+	srgb_table_init(lut);
 	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	if (i >= effective_count(d_count, max_count, max_count)) return;
 	const uint32_t *w = reinterpret_cast<const uint32_t *>(rec + i);
 	float q[kTrainFloats];
-	unpack_input(s, load_packed(w + 6), q);
+	unpack_input(s, load_packed(w + 6), lut, q);
 	// nrc_gradient.comp:30-33: the target is the record's bias
 	q[14] = __uint_as_float(w[0]);
 	q[15] = __uint_as_float(w[1]);

@@ -306,8 +306,9 @@ def group_workload(torch, v, wl, q_host, frames: int) -> dict:
             "graph_ms_per_frame": out["graph"], "graph_queries_per_s": total_q / (out["graph"] / 1e3),
             "grouped_vs_sequential": out["sequential"] / out["grouped"],
             "graph_vs_sequential": out["sequential"] / out["graph"],
-            "autotuned": {"grad_chunks": tuned_grad["best"],
+            "autotuned": {"grad_chunks": tuned_grad["best"], "grad_chunks_per_cache": tuned_grad["per_cache"],
                           "grad_ms_per_step": {str(k): round(x, 4) for k, x in tuned_grad["ms_per_step"].items()},
+                          "grad_ms_per_step_per_cache_tuned": round(tuned_grad["tuned_ms_per_step"], 4),
                           "infer_workgroups_per_cu": tuned_infer["best"],
                           "infer_ms": {str(k): round(x, 4) for k, x in tuned_infer["ms_per_launch"].items()}}}
 
@@ -435,6 +436,21 @@ def main() -> int:
     torch.cuda.synchronize()
     f8_ms = np.array([a.elapsed_time(b) for a, b in f8ev])
     f8_err = float(torch.linalg.vector_norm((out - out16).double()) / torch.linalg.vector_norm(out16.double()))
+    # both against an FP64 forward on the GPU (torch float64 matmuls on the library's FP32 features and the
+    # FP32 use-weights, which equal the master weights after the last step wrote them) over a 65,536-query sample
+    ns = 65536
+    feats = torch.zeros((ns, 64), dtype=torch.float32, device=dev)
+    v.encode(dq[:ns], ns, feats, encoding=v.ENCODING_SINCOS6, stream=stream)
+    w64 = torch.from_numpy(st.get_state()["weight"].astype(np.float64)).to(dev)
+    a = feats.double()
+    for l in range(5):
+        a = torch.relu(a @ w64[l * 4096:(l + 1) * 4096].reshape(64, 64).T)
+    ref64 = torch.relu(a @ w64[5 * 4096:].reshape(3, 64).T)
+    torch.cuda.synchronize()
+
+    def err64(y):
+        return float(torch.linalg.vector_norm(y[:ns].double() - ref64) / torch.linalg.vector_norm(ref64))
+    f8_err64, f16_err64 = err64(out), err64(out16)
 
     # plain FP32 inference at the same size (the plain method)
     plain_ms = []
@@ -492,6 +508,8 @@ def main() -> int:
     f8_tflops = INFER_FLOP_PER_QUERY * QUERIES / f8_s / 1e12
     res["fp8"] = {"infer_queries_per_s": QUERIES / f8_s, "infer_ms_avg": f8_s * 1e3,
                   "speedup_vs_f16": infer_avg_s / f8_s, "rel_err_vs_f16": f8_err,
+                  "rel_err_vs_fp64": f8_err64, "f16_rel_err_vs_fp64": f16_err64,
+                  "error_sample": "65,536 queries, FP64 forward with the FP32 use-weights on the GPU",
                   "achieved_TFLOPs": f8_tflops, "peak_TFLOPs": PEAK_F8_TFLOPS, "frac": f8_tflops / PEAK_F8_TFLOPS}
     res["epilogue"] = epilogue
     log("record front-end ...")

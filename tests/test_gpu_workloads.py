@@ -66,8 +66,10 @@ def test_caches16_grouped_equals_alone_equals_graph_and_the_oracle():
     grouped, alone, graphed = caches16_set(spec, pool), caches16_set(spec, pool), caches16_set(spec, pool)
     tuned = autotune.tune_grad_chunks(torch, grouped)
     autotune.tune_infer_workgroups(torch, grouped)
-    for e in alone + graphed:
-        e["cache"].set_grad_chunks(tuned["best"])
+    assert len(tuned["per_cache"]) == 16
+    for i, e in enumerate(alone):
+        e["cache"].set_grad_chunks(tuned["per_cache"][i])
+        graphed[i]["cache"].set_grad_chunks(tuned["per_cache"][i])
     graph = v.GroupGraph(graphed)
     for _ in range(3):
         v.group_frame(grouped)

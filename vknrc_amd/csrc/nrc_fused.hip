@@ -31,8 +31,11 @@ namespace {
 template <int H, int ENC, bool F16IN, int TILES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void fused_infer_kernel(const InferGroup grp) {
 	__shared__ __attribute__((aligned(16))) half8_t sX[4][8 * 64];
+	__shared__ uint2 sSel[4][8]; // each wave's one-blob selector table
 
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	oneblob_sel_init(sSel[wave], lane);
+	wave_lds_sync();
 	const int g = lane >> 4, r = lane & 15;
 	const uint32_t nW = gridDim.x * 4, gw = blockIdx.x * 4 + wave;
 	const uint32_t t0 = (uint32_t)((uint64_t)grp.total_tiles * gw / nW);
@@ -88,12 +91,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 		// encode one record per lane straight into the B-fragment layout of this wave's slot
 		auto encode_tile = [&](uint32_t base, const float *rec) __attribute__((always_inline)) {
 			const uint32_t n = base + lane;
-			auto emit = [&](int p, float f0, float f1, float f2, float f3) __attribute__((always_inline)) {
-				const int c = p >> 3, gg = p & 3, hi = (p >> 2) & 1;
-				half4_t h;
-				h[0] = (_Float16)f0; h[1] = (_Float16)f1; h[2] = (_Float16)f2; h[3] = (_Float16)f3;
-				sx4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = h;
-			};
+			const SlotEmit emit{sx4, et, es, sSel[wave]};
 			if constexpr (F16IN) {
 				if (n < count) {
 					const half4_t *src = reinterpret_cast<const half4_t *>(grp.fin + (size_t)n * kWidth);
@@ -108,7 +106,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 				}
 			} else {
 				if (n < count) {
-					encode_quads<ENC>(rec, emit);
+					encode_quads<ENC, true>(rec, emit);
 				} else {
 #pragma unroll
 					for (int p = 0; p < 16; ++p) emit(p, 0.0f, 0.0f, 0.0f, 0.0f);
@@ -203,16 +201,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 	// slot never alias the layers' loads from the other
 	__shared__ __attribute__((aligned(16))) half8_t sX0[4][8 * 64];
 	__shared__ __attribute__((aligned(16))) half8_t sX1[4][8 * 64];
+	__shared__ uint2 sSel[4][8]; // each wave's one-blob selector table
 
 	// the wave index and everything derived from it are wave-uniform: in SGPRs
 	const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	oneblob_sel_init(sSel[wave], lane);
+	wave_lds_sync();
 	const int g = lane >> 4, r = lane & 15;
 	const uint32_t nW = gridDim.x * 4, gw = blockIdx.x * 4 + wave;
 	const uint32_t t0 = (uint32_t)((uint64_t)grp.total_tiles * gw / nW);
 	const uint32_t t1 = (uint32_t)((uint64_t)grp.total_tiles * (gw + 1) / nW);
 	half8_t *slot0 = sX0[wave], *slot1 = sX1[wave];
 	const int et = lane >> 4, es = lane & 15; // this lane's record: N-tile et, column es
-	const float lo = grp.clamp0 ? 0.0f : -__builtin_huge_valf();
+	// the output clamp max(o, 0) as a signed integer max of the bits: every value with
+	// the sign bit set (negative, -0, negative NaN) becomes +0, the rest pass (a
+	// floating-point max would first canonicalize the MFMA result, an extra instruction
+	// per value); without the clamp the floor INT_MIN passes every value
+	const int lo_bits = grp.clamp0 ? 0 : INT_MIN;
 
 	for (uint32_t i = 0; i < grp.n_items; ++i) {
 		const InferItem &it = grp.item[i];
@@ -253,13 +258,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 			}
 		};
 		auto emitter = [&](half8_t *slot) __attribute__((always_inline)) {
-			half4_t *s4 = reinterpret_cast<half4_t *>(slot);
-			return [=](int p, float f0, float f1, float f2, float f3) __attribute__((always_inline)) {
-				const int c = p >> 3, gg = p & 3, hi = (p >> 2) & 1;
-				half4_t h;
-				h[0] = (_Float16)f0; h[1] = (_Float16)f1; h[2] = (_Float16)f2; h[3] = (_Float16)f3;
-				s4[((et * 2 + c) * 64 + gg * 16 + es) * 2 + hi] = h;
-			};
+			return SlotEmit{reinterpret_cast<half4_t *>(slot), et, es, sSel[wave]};
 		};
 		constexpr int NP = encode_pieces<ENC>();
 		// piece k of the next tile's encoding goes after the MFMAs of step s = k * steps / NP
@@ -301,9 +300,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 				for (int c = 0; c < 2; ++c) o = mfma(Ao[c], B[c], o);
 				piece_at(std::integral_constant<int, nt * (H + 1) + H>{}, rec, nxt);
 				uint3_t v;
-				v[0] = __float_as_uint(fmaxf(o[0], lo));
-				v[1] = __float_as_uint(fmaxf(o[1], lo));
-				v[2] = __float_as_uint(fmaxf(o[2], lo));
+#pragma unroll
+				for (int e = 0; e < 3; ++e) {
+					const int b = __float_as_int(o[e]);
+					v[e] = (uint32_t)(b > lo_bits ? b : lo_bits);
+				}
 				const uint32_t off = g == 0 ? (uint32_t)((nt * 16 + r) * kOut * 4) : 0x7ffffff0u;
 				__builtin_amdgcn_raw_buffer_store_b96(v, rs, off, 0, 0);
 			});
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 		float ra[kQueryFloats], rb[kQueryFloats];
 		load_query(lt0, ra);
 		load_query(lt0 + 1, rb);
-		encode_quads<ENC>(ra, emitter(slot0));
+		encode_quads<ENC, true>(ra, emitter(slot0));
 		wave_lds_sync();
 		for (uint32_t t = lt0; t < lt1; t += 2) {
 			load_query(t + 2, ra);

@@ -107,6 +107,56 @@ __device__ __forceinline__ void oneblob4_knots(float x, float *o) {
 	for (int b = 0; b < 4; ++b) o[b] = c[b + 1] - c[b];
 }
 
+// The same one-blob quad with two knots, packed to FP16.  With j = floor(4x) and
+// f = 4x - j (exact), only knots j and j+1 have |u| < 1: c_j = C(-f), c_{j+1} =
+// C(1 - f), every knot below j is 0 and every one above j+1 is 1.  So the quad holds
+// c_j, c_{j+1} - c_j and 1 - c_{j+1} at positions j-1, j and j+1 and zeros elsewhere.
+// The knot arguments equal the five-knot ones bit for bit (f and 1 - f are exact for
+// 4x in [0, 4]), so the two forms agree except where the five-knot form evaluates a
+// saturated knot (|u| just above 1) to one FP32 ulp short of 0 or 1: there it gives
+// a feature of 6e-8 (the smallest FP16 subnormal) and this form the exact 0.  An
+// exhaustive check over every float in [0, 1] finds 1,786 such inputs.
+// The placement is two byte permutes of (c_j, c_{j+1} - c_j | 1 - c_{j+1}, 0) with
+// selectors from a 6-entry table indexed by j + 1 (selector byte 0x0C is a zero
+// byte); 4x is clamped to [-1, 5), where the quad is exact (it is zero from 4x = 5
+// on and has only its first feature from 4x = -1 on).
+constexpr uint32_t kOneBlobSel[6][2] = {
+    {0x0C0C0504u, 0x0C0C0C0Cu}, // j = -1: (1 - c1, 0, 0, 0)
+    {0x05040302u, 0x0C0C0C0Cu}, // j = 0:  (c1 - c0, 1 - c1, 0, 0)
+    {0x03020100u, 0x0C0C0504u}, // j = 1:  (c0, c1 - c0, 1 - c1, 0)
+    {0x01000C0Cu, 0x05040302u}, // j = 2:  (0, c0, c1 - c0, 1 - c1)
+    {0x0C0C0C0Cu, 0x03020100u}, // j = 3:  (0, 0, c0, c1 - c0)
+    {0x0C0C0C0Cu, 0x01000C0Cu}, // j = 4:  (0, 0, 0, c0)
+};
+
+// writes the selector table into tab[0..5] (lanes 0..5 of the calling wave)
+__device__ __forceinline__ void oneblob_sel_init(uint2 *tab, int lane) {
+	if (lane < 6) tab[lane] = make_uint2(kOneBlobSel[lane][0], kOneBlobSel[lane][1]);
+}
+
+__device__ __forceinline__ float quartic_cdf(float u) {
+	const float u2 = u * u;
+	float t = fmaf(u2, 0.1875f, -0.625f);
+	t = fmaf(u2, t, 0.9375f);
+	return fminf(fmaxf(fmaf(u, t, 0.5f), 0.0f), 1.0f);
+}
+
+__device__ __forceinline__ uint32_t pack_half2_bits(float a, float b) {
+	const float2_t f = {a, b};
+	return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, half2_t));
+}
+
+__device__ __forceinline__ uint2 oneblob4_packed(float x, const uint2 *sel) {
+	const float x4 = __builtin_amdgcn_fmed3f(4.0f * x, -1.0f, 0x1.3ffffep+2f);
+	const float j = __builtin_floorf(x4);
+	const float f = x4 - j;
+	const float c0 = quartic_cdf(-f), c1 = quartic_cdf(1.0f - f);
+	const uint32_t lo = pack_half2_bits(c0, c1 - c0), hi = pack_half2_bits(1.0f - c1, 0.0f);
+	// j + 1 is in [0, 5] (x4 is clamped; the conversion of a NaN gives 0)
+	const uint2 s = sel[(int)j + 1];
+	return make_uint2(__builtin_amdgcn_perm(hi, lo, s.x), __builtin_amdgcn_perm(hi, lo, s.y));
+}
+
 template <int ENC> __device__ __forceinline__ void axis12_infer(float p, float *o) {
 	if constexpr (ENC == NRC_ENCODING_SINCOS6) {
 		float sv, cv;
@@ -130,7 +180,9 @@ template <int ENC> __device__ __forceinline__ void axis12_infer(float p, float *
 	}
 }
 // Emits the 16 feature quads (features 4p..4p+3) of one record; Emit(p, f0..f3).
-template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(const float *rec, Emit &&emit) {
+// PACKED: the one-blob quads come from oneblob4_packed through Emit.bits(p, halves)
+// with the selector table Emit.sel (the FP16 kernels); otherwise from the five knots.
+template <int ENC, bool PACKED = false, typename Emit> __device__ __forceinline__ void encode_quads(const float *rec, Emit &&emit) {
 	if constexpr (ENC == NRC_ENCODING_UV_SINCOS16) {
 #pragma unroll
 		for (int a = 0; a < 2; ++a) {
@@ -162,25 +214,26 @@ template <int ENC, typename Emit> __device__ __forceinline__ void encode_quads(c
 			emit(3 * a + 1, t[4], t[5], t[6], t[7]);
 			emit(3 * a + 2, t[8], t[9], t[10], t[11]);
 		}
-		float b[4];
-		oneblob4_knots(rec[3], b);
-		emit(9, b[0], b[1], b[2], b[3]);
-		oneblob4_knots(rec[4], b);
-		emit(10, b[0], b[1], b[2], b[3]);
-		oneblob4_knots(rec[5], b);
-		emit(11, b[0], b[1], b[2], b[3]);
-		oneblob4_knots(rec[6], b);
-		emit(12, b[0], b[1], b[2], b[3]);
-		oneblob4_knots(1.0f - expf(-rec[7]), b);
-		emit(13, b[0], b[1], b[2], b[3]);
+		if constexpr (PACKED) {
+#pragma unroll
+			for (int q = 0; q < 5; ++q) emit.bits(9 + q, oneblob4_packed(q == 4 ? 1.0f - expf(-rec[7]) : rec[3 + q], emit.sel));
+		} else {
+			float b[4];
+#pragma unroll
+			for (int q = 0; q < 5; ++q) {
+				oneblob4_knots(q == 4 ? 1.0f - expf(-rec[7]) : rec[3 + q], b);
+				emit(9 + q, b[0], b[1], b[2], b[3]);
+			}
+		}
 		emit(14, rec[8], rec[9], rec[10], rec[11]);
 		emit(15, rec[12], rec[13], 1.0f, 1.0f);
 	}
 }
 
 
-// The same quads in pieces, so that the encoding of one tile can be spread over
-// the layers of another: piece P of encode_pieces<ENC>() emits
+// The same quads in pieces (the one-blob quads packed, as encode_quads<ENC, true>), so
+// that the encoding of one tile can be spread over the layers of another: piece P of
+// encode_pieces<ENC>() emits
 //   TRI12 / SINCOS6:  P = 0..2 the 12 features of position axis P (quads 3P..3P+2),
 //                     P = 3..7 one one-blob quad each (quads 9..13), P = 8 quads 14, 15;
 //   UV_SINCOS16:      P = 0..7 the sin and cos quads of octaves 4(P&3).. of axis P>>2.
@@ -211,14 +264,28 @@ template <int ENC, int P, typename Emit> __device__ __forceinline__ void encode_
 		emit(3 * P + 1, t[4], t[5], t[6], t[7]);
 		emit(3 * P + 2, t[8], t[9], t[10], t[11]);
 	} else if constexpr (P < 8) {
-		float b[4];
-		oneblob4_knots(P == 7 ? 1.0f - expf(-rec[7]) : rec[P], b);
-		emit(6 + P, b[0], b[1], b[2], b[3]);
+		emit.bits(6 + P, oneblob4_packed(P == 7 ? 1.0f - expf(-rec[7]) : rec[P], emit.sel));
 	} else {
 		emit(14, rec[8], rec[9], rec[10], rec[11]);
 		emit(15, rec[12], rec[13], 1.0f, 1.0f);
 	}
 }
+
+// Writes feature quads of lane (et, es)'s record into a wave's LDS slot in the
+// B-fragment layout: quad p holds features 4p..4p+3, K-chunk p >> 3, lane group
+// p & 3, half (p >> 2) & 1 of the lane's 8 elements.
+struct SlotEmit {
+	half4_t *s4;
+	int et, es;
+	const uint2 *sel; // the one-blob selector table
+	__device__ __forceinline__ int at(int p) const { return (((et * 2 + (p >> 3)) * 64 + (p & 3) * 16 + es) * 2 + ((p >> 2) & 1)); }
+	__device__ __forceinline__ void operator()(int p, float f0, float f1, float f2, float f3) const {
+		half4_t h;
+		h[0] = (_Float16)f0; h[1] = (_Float16)f1; h[2] = (_Float16)f2; h[3] = (_Float16)f3;
+		s4[at(p)] = h;
+	}
+	__device__ __forceinline__ void bits(int p, uint2 v) const { s4[at(p)] = __builtin_bit_cast(half4_t, v); }
+};
 
 // feature index of element j of a B operand held by lane group g for K-chunk c
 __device__ __forceinline__ int bfeat(int c, int g, int j) { return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4); }

@@ -117,21 +117,23 @@ __device__ __forceinline__ void oneblob4_knots(float x, float *o) {
 // a feature of 6e-8 (the smallest FP16 subnormal) and this form the exact 0.  An
 // exhaustive check over every float in [0, 1] finds 1,786 such inputs.
 // The placement is two byte permutes of (c_j, c_{j+1} - c_j | 1 - c_{j+1}, 0) with
-// selectors from a 6-entry table indexed by j + 1 (selector byte 0x0C is a zero
-// byte); 4x is clamped to [-1, 5), where the quad is exact (it is zero from 4x = 5
-// on and has only its first feature from 4x = -1 on).
-constexpr uint32_t kOneBlobSel[6][2] = {
+// selectors from an 8-entry table indexed by clamp(j, -2, 5) + 2 (selector byte 0x0C
+// is a zero byte): j <= -2 (4x < -1) and j >= 5 (4x >= 5) give the all-zero quad,
+// which is exact there, so no bound on x needs a register.
+constexpr uint32_t kOneBlobSel[8][2] = {
+    {0x0C0C0C0Cu, 0x0C0C0C0Cu}, // j <= -2: zeros
     {0x0C0C0504u, 0x0C0C0C0Cu}, // j = -1: (1 - c1, 0, 0, 0)
     {0x05040302u, 0x0C0C0C0Cu}, // j = 0:  (c1 - c0, 1 - c1, 0, 0)
     {0x03020100u, 0x0C0C0504u}, // j = 1:  (c0, c1 - c0, 1 - c1, 0)
     {0x01000C0Cu, 0x05040302u}, // j = 2:  (0, c0, c1 - c0, 1 - c1)
     {0x0C0C0C0Cu, 0x03020100u}, // j = 3:  (0, 0, c0, c1 - c0)
     {0x0C0C0C0Cu, 0x01000C0Cu}, // j = 4:  (0, 0, 0, c0)
+    {0x0C0C0C0Cu, 0x0C0C0C0Cu}, // j >= 5: zeros
 };
 
-// writes the selector table into tab[0..5] (lanes 0..5 of the calling wave)
+// writes the selector table into tab[0..7] (lanes 0..7 of the calling wave)
 __device__ __forceinline__ void oneblob_sel_init(uint2 *tab, int lane) {
-	if (lane < 6) tab[lane] = make_uint2(kOneBlobSel[lane][0], kOneBlobSel[lane][1]);
+	if (lane < 8) tab[lane] = make_uint2(kOneBlobSel[lane][0], kOneBlobSel[lane][1]);
 }
 
 __device__ __forceinline__ float quartic_cdf(float u) {
@@ -147,13 +149,14 @@ __device__ __forceinline__ uint32_t pack_half2_bits(float a, float b) {
 }
 
 __device__ __forceinline__ uint2 oneblob4_packed(float x, const uint2 *sel) {
-	const float x4 = __builtin_amdgcn_fmed3f(4.0f * x, -1.0f, 0x1.3ffffep+2f);
+	const float x4 = 4.0f * x;
 	const float j = __builtin_floorf(x4);
 	const float f = x4 - j;
 	const float c0 = quartic_cdf(-f), c1 = quartic_cdf(1.0f - f);
 	const uint32_t lo = pack_half2_bits(c0, c1 - c0), hi = pack_half2_bits(1.0f - c1, 0.0f);
-	// j + 1 is in [0, 5] (x4 is clamped; the conversion of a NaN gives 0)
-	const uint2 s = sel[(int)j + 1];
+	int k = (int)j; // the conversion saturates (and gives 0 for a NaN)
+	k = k < -2 ? -2 : (k > 5 ? 5 : k);
+	const uint2 s = sel[k + 2];
 	return make_uint2(__builtin_amdgcn_perm(hi, lo, s.x), __builtin_amdgcn_perm(hi, lo, s.y));
 }
 

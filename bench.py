@@ -43,10 +43,10 @@ QUERY_BYTES = 56 + 12             # 14-float record in, RGB out
 PEAK_F16_TFLOPS = 2500.0
 PEAK_F8_TFLOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per fused_infer_kernel launch at this workload: rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0)
-# KiB per dispatch, scaled by a 256 MiB copy kernel measured in the same runs (bench.py --copy-calibration);
-# the commands and readings are in profiles/fused_infer_2073600q_hbm_traffic.md
-INFER_TRAFFIC_BYTES = (57007.0 * 2.0 + 24300.5) * 1024
+# HBM bytes per fused_infer_pipe_kernel launch at this workload: rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE
+# (x1.0) KiB per dispatch, scaled by a 256 MiB copy kernel measured in the same runs (bench.py
+# --copy-calibration); the commands and readings are in profiles/fused_infer_2073600q_hbm_traffic.md
+INFER_TRAFFIC_BYTES = (57005.4 * 2.0 + 25325.5) * 1024
 
 
 def log(*a):

@@ -244,8 +244,20 @@ def frontend_workload(torch, v, wl, reps: int) -> dict:
     order = np.argsort(rec["packed_input"]["primitive_id"], kind="stable")
     d_rec_sorted = torch.from_numpy(rec[order].copy().view(np.uint8)).to("cuda")
     runs["unpack_eval_sorted"] = (lambda: v.unpack_eval_records(ds, d_rec_sorted, NQ, q, stream=stream), NQ * (20 + 56))
+    # the same scene with its primitives packed one to a 64-byte record (nrc_scene_pack_primitives)
+    dsp = n.DeviceScene(torch, s["vertices"], s["vertex_indices"], s["texcoords"], s["texcoord_indices"],
+                        s["materials"], s["material_ids"], s["transforms"], s["textures"])
+    dsp.pack_primitives(stream=stream)
+    P = len(s["material_ids"])
+    runs["pack_primitives"] = (lambda: dsp.pack_primitives(stream=stream), P * (24 + 4 + 64) + P * 3 * (12 + 8))
+    runs["unpack_eval_packed"] = (lambda: v.unpack_eval_records(dsp, d_rec, NQ, q, stream=stream), NQ * (20 + 56))
+    runs["unpack_eval_sorted_packed"] = (lambda: v.unpack_eval_records(dsp, d_rec_sorted, NQ, q, stream=stream),
+                                         NQ * (20 + 56))
+    runs["unpack_train_packed"] = (lambda: v.unpack_train_records(dsp, d_tr, T, t17, stream=stream), T * (40 + 68))
     res = {"scene": "65,536 triangles, 8 instances, 32 materials, 8 textures 256x256 RGBA8 sRGB; records on "
-                    "uniformly random primitives (unpack_eval_sorted: the same records sorted by primitive)"}
+                    "uniformly random primitives (unpack_eval_sorted: the same records sorted by primitive; "
+                    "*_packed: the scene's primitives packed one to a 64-byte record by nrc_scene_pack_primitives, "
+                    "whose own time is pack_primitives)"}
     for name, (fn, nbytes) in runs.items():
         for _ in range(3):
             fn()

@@ -305,6 +305,16 @@ typedef struct nrc_texture {
 /* The scene buffers of Scene.glsl:26-47 (device pointers).  A vertex of primitive
  * p is d_vertices[d_vertex_indices[3p + k]] transformed by its instance's
  * d_transforms[i] (3 x 4 floats, row j giving world_j = row_j . (x, y, z, 1)). */
+/* One primitive in one 64-byte record: its corners in object space (before the
+ * instance transform), their texture coordinates and its material id, as the index
+ * arrays of nrc_scene name them.  No counterpart in the reference, which gathers them
+ * through the indices per record (Scene.glsl:50-62); see nrc_scene_pack_primitives. */
+typedef struct nrc_scene_primitive {
+	float corners[3][3];
+	float texcoords[3][2];
+	uint32_t material_id;
+} nrc_scene_primitive;
+
 typedef struct nrc_scene {
 	const float *d_vertices;            /* [vertices][3] */
 	const uint32_t *d_vertex_indices;   /* [primitives * 3] */
@@ -315,7 +325,18 @@ typedef struct nrc_scene {
 	const float *d_transforms;          /* [instances][3][4] */
 	const nrc_texture *d_textures;      /* device array of texture_count descriptors, or NULL */
 	uint32_t texture_count;
+	/* optional: the primitives packed by nrc_scene_pack_primitives (NULL: the unpack
+	 * gathers through the index arrays); the unpacked records are the same bit for bit */
+	const nrc_scene_primitive *d_primitives;
 } nrc_scene;
+
+/* Packs primitives [0, primitive_count) of the scene into d_out [primitive_count],
+ * 64-byte aligned (device memory from the caller), once per scene: unpacking a record
+ * then reads one 64-byte record for its primitive instead of about nine scattered
+ * cache lines of indices, corners, texcoords and material ids.  No counterpart in the
+ * reference; for callers that unpack many records on one scene. */
+nrc_status nrc_scene_pack_primitives(const nrc_scene *scene, uint32_t primitive_count, nrc_scene_primitive *d_out,
+                                     void *stream);
 
 /* UnpackNRCInput (NRCRecord.glsl:98-125) of the first min(*d_count, max_count)
  * eval records (d_count NULL: max_count) into d_queries [max_count]; the records'

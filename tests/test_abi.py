@@ -119,3 +119,58 @@ def test_workload_generators():
     uv = np.array([[0.5 / 256, 0.5 / 256], [100.5 / 256, 7.5 / 256]])
     assert np.allclose(wl.sample_bilinear(img, uv), img[[0, 7], [0, 100]], atol=1e-6)
     assert ir.shape == (100, 17)
+
+
+def _header_layout():
+    """sizeof and offsetof of the ABI structs as a C99 compiler lays out include/nrc.h."""
+    import os
+    import subprocess
+    import tempfile
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    fields = {"nrc_cache_desc": [], "nrc_optimizer_state": [], "nrc_optimizer_entry": [], "nrc_group_item": [],
+              "nrc_screen": [], "nrc_texture": [], "nrc_material": ["diffuse_texture_id", "specular_texture_id",
+                                                                     "roughness"],
+              "nrc_packed_input": [], "nrc_eval_record": ["packed_input"],
+              "nrc_wire_train_record": ["factor", "packed_input"], "nrc_traced_path": [],
+              "nrc_scene": ["d_textures", "texture_count", "d_primitives"],
+              "nrc_scene_primitive": ["texcoords", "material_id"], "nrc_query": [], "nrc_train_record": []}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "nrc.h"', "int main(void) {"]
+    for t, fs in fields.items():
+        lines.append(f'  printf("{t} %zu\\n", sizeof({t}));')
+        for f in fs:
+            lines.append(f'  printf("{t}.{f} %zu\\n", offsetof({t}, {f}));')
+    lines += ["  return 0;", "}"]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, src, "-o", exe], check=True,
+                       capture_output=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return {k: int(n) for k, n in (l.split() for l in out.splitlines())}
+
+
+def test_python_mirrors_match_the_c_layout_of_the_header():
+    h = _header_layout()
+    n = v.nrc
+    assert C.sizeof(n.CacheDesc) == h["nrc_cache_desc"]
+    assert C.sizeof(n.OptimizerState) == h["nrc_optimizer_state"]
+    assert h["nrc_optimizer_entry"] == 16
+    assert C.sizeof(n.GroupItem) == h["nrc_group_item"]
+    assert C.sizeof(n.Screen) == h["nrc_screen"]
+    assert C.sizeof(n.Texture) == h["nrc_texture"]
+    assert C.sizeof(n.Scene) == h["nrc_scene"]
+    for f in ("d_textures", "texture_count", "d_primitives"):
+        assert getattr(n.Scene, f).offset == h[f"nrc_scene.{f}"], f
+    assert h["nrc_scene_primitive"] == 64 and h["nrc_scene_primitive.texcoords"] == 36
+    assert h["nrc_scene_primitive.material_id"] == 60
+    assert n.PACKED_INPUT.itemsize == h["nrc_packed_input"] == 16
+    assert n.EVAL_RECORD.itemsize == h["nrc_eval_record"] == 20
+    assert n.EVAL_RECORD.fields["packed_input"][1] == h["nrc_eval_record.packed_input"]
+    assert n.WIRE_TRAIN_RECORD.itemsize == h["nrc_wire_train_record"] == 40
+    for f in ("factor", "packed_input"):
+        assert n.WIRE_TRAIN_RECORD.fields[f][1] == h[f"nrc_wire_train_record.{f}"], f
+    assert n.MATERIAL.itemsize == h["nrc_material"]
+    for f in ("diffuse_texture_id", "specular_texture_id", "roughness"):
+        assert n.MATERIAL.fields[f][1] == h[f"nrc_material.{f}"], f
+    assert n.TRACED_PATH.itemsize == h["nrc_traced_path"]
+    assert h["nrc_query"] == 56 and h["nrc_train_record"] == 68

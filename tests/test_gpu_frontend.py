@@ -76,6 +76,49 @@ def test_unpack_eval_and_train_records_match_oracle(scene):
     assert np.array_equal(t[:, 14:17], tr["bias"])
 
 
+def test_packed_primitives_unpack_the_same_bits(scene):
+    """nrc_scene_pack_primitives: the 64-byte records hold the indexed corners, texcoords and material
+    ids bit for bit, and unpacking through them gives the same eval and train records as the gather
+    through the index arrays."""
+    s, _ = scene
+    ds = n.DeviceScene(torch, s["vertices"], s["vertex_indices"], s["texcoords"], s["texcoord_indices"],
+                       s["materials"], s["material_ids"], s["transforms"], s["textures"])
+    ds.pack_primitives()
+    P = len(s["material_ids"])
+    packed = ds.primitives.cpu().numpy().view(np.float32).reshape(P, 16)
+    vi = np.asarray(s["vertex_indices"], np.int64).reshape(P, 3)
+    ti = np.asarray(s["texcoord_indices"], np.int64).reshape(P, 3)
+    assert np.array_equal(packed[:, :9].reshape(P, 3, 3), np.asarray(s["vertices"], np.float32)[vi])
+    assert np.array_equal(packed[:, 9:15].reshape(P, 3, 2), np.asarray(s["texcoords"], np.float32)[ti])
+    assert np.array_equal(packed[:, 15].view(np.uint32), np.asarray(s["material_ids"], np.uint32))
+    rng = np.random.default_rng(802)
+    nrec = 60001
+    recs = np.zeros(nrec, n.EVAL_RECORD)
+    recs["packed_input"] = wl.make_packed_inputs(s, nrec, rng)
+    drec = dev_bytes(recs)
+    outs = []
+    for on in (False, True):
+        ds.use_packed(on)
+        o = torch.zeros((nrec, 14), dtype=torch.float32, device=DEV)
+        v.unpack_eval_records(ds, drec, nrec, o)
+        outs.append(o.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    tr = np.zeros(7000, n.WIRE_TRAIN_RECORD)
+    tr["bias"] = rng.lognormal(0, 1, (7000, 3))
+    tr["packed_input"] = recs["packed_input"][:7000]
+    dtr, touts = dev_bytes(tr), []
+    for on in (False, True):
+        ds.use_packed(on)
+        o = torch.zeros((7000, 17), dtype=torch.float32, device=DEV)
+        v.unpack_train_records(ds, dtr, 7000, o)
+        touts.append(o.cpu().numpy())
+    assert np.array_equal(touts[0].view(np.uint32), touts[1].view(np.uint32))
+    # a misaligned packed table is refused before anything launches
+    ds.c.d_primitives = ds.primitives.data_ptr() + 4
+    with pytest.raises(n.NRCError):
+        v.unpack_eval_records(ds, drec, nrec, torch.zeros((nrec, 14), dtype=torch.float32, device=DEV))
+
+
 def test_composite_records_bit_exact(scene):
     s, _ = scene
     rng = np.random.default_rng(802)

@@ -601,17 +601,33 @@ nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32
 } // extern "C"
 
 namespace {
+static_assert(sizeof(nrc_scene_primitive) == 64, "a packed primitive is one 64-byte record");
+
 nrc_status check_scene(const nrc_scene *s) {
 	if (!s) return fail(NRC_ERROR_INVALID_ARGUMENT, "scene is NULL");
 	if (!s->d_vertices || !s->d_vertex_indices || !s->d_texcoords || !s->d_texcoord_indices || !s->d_materials ||
 	    !s->d_material_ids || !s->d_transforms)
 		return fail(NRC_ERROR_INVALID_ARGUMENT, "scene buffer NULL");
 	if (s->texture_count && !s->d_textures) return fail(NRC_ERROR_INVALID_ARGUMENT, "texture table NULL");
+	if (reinterpret_cast<uintptr_t>(s->d_primitives) % 16u != 0u)
+		return fail(NRC_ERROR_INVALID_ARGUMENT, "packed primitives not 16-byte aligned");
 	return NRC_OK;
 }
 } // namespace
 
 extern "C" {
+
+nrc_status nrc_scene_pack_primitives(const nrc_scene *scene, uint32_t primitive_count, nrc_scene_primitive *d_out,
+                                     void *stream) {
+	if (primitive_count == 0) return NRC_OK;
+	nrc_status st = check_scene(scene);
+	if (st != NRC_OK) return st;
+	if (!d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (reinterpret_cast<uintptr_t>(d_out) % 64u != 0u) return fail(NRC_ERROR_INVALID_ARGUMENT, "d_out not 64-byte aligned");
+	hipError_t e = nrc::launch_pack_primitives(*scene, primitive_count, d_out, reinterpret_cast<hipStream_t>(stream));
+	if (e != hipSuccess) return hip_fail(e, "nrc_scene_pack_primitives");
+	return NRC_OK;
+}
 
 nrc_status nrc_unpack_eval_records(const nrc_scene *scene, const nrc_eval_record *d_records, const uint32_t *d_count,
                                    uint32_t max_count, nrc_query *d_queries, void *stream) {

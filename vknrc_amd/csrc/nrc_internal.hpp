@@ -165,6 +165,7 @@ hipError_t launch_zero_words(uint32_t *dst, size_t words, hipStream_t stream);
 
 // nrc_records.hip: the record front-end, the wire-record epilogue, the path
 // suffix sums and the screen pass
+hipError_t launch_pack_primitives(const nrc_scene &s, uint32_t n, nrc_scene_primitive *out, hipStream_t stream);
 hipError_t launch_unpack_eval(const nrc_scene &s, const nrc_eval_record *rec, const uint32_t *d_count,
                               uint32_t max_count, float *queries, hipStream_t stream);
 hipError_t launch_unpack_train(const nrc_scene &s, const nrc_wire_train_record *rec, const uint32_t *d_count,

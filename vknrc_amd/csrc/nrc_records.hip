@@ -20,16 +20,6 @@ __device__ __forceinline__ void unorm2x16(uint32_t v, float *o) {
 	o[1] = (float)(v >> 16) / 65535.0f;
 }
 
-// GetSceneVertex (Scene.glsl:50-53): (x, y, z, 1) times the instance's 3x4 rows
-__device__ __forceinline__ void scene_vertex(const nrc_scene &s, uint32_t inst, uint32_t prim, uint32_t k, float *o) {
-	const uint32_t vi = s.d_vertex_indices[prim * 3u + k];
-	const float x = s.d_vertices[(size_t)vi * 3], y = s.d_vertices[(size_t)vi * 3 + 1],
-	            z = s.d_vertices[(size_t)vi * 3 + 2];
-	const float *T = s.d_transforms + (size_t)inst * 12;
-#pragma unroll
-	for (int j = 0; j < 3; ++j) o[j] = T[4 * j] * x + T[4 * j + 1] * y + T[4 * j + 2] * z + T[4 * j + 3];
-}
-
 __device__ __forceinline__ float srgb_decode(uint32_t c8) {
 	const float c = (float)c8 / 255.0f;
 	return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
@@ -90,14 +80,57 @@ __device__ __forceinline__ void material_color(const nrc_scene &s, const float *
 	}
 }
 
+// A primitive's corners (object space), their texture coordinates and its material:
+// gathered through the index arrays (Scene.glsl:50-62), or read from the primitive's
+// 64-byte packed record (nrc_scene_pack_primitives) as four 16-byte loads.
+struct PrimitiveData {
+	float c[3][3], t[3][2];
+	uint32_t material;
+};
+
+template <bool PACKED> __device__ __forceinline__ PrimitiveData load_primitive(const nrc_scene &s, uint32_t prim) {
+	PrimitiveData d;
+	if constexpr (PACKED) {
+		const float4 *r = reinterpret_cast<const float4 *>(s.d_primitives + prim);
+		const float4 a = r[0], b = r[1], c = r[2], e = r[3];
+		d.c[0][0] = a.x; d.c[0][1] = a.y; d.c[0][2] = a.z;
+		d.c[1][0] = a.w; d.c[1][1] = b.x; d.c[1][2] = b.y;
+		d.c[2][0] = b.z; d.c[2][1] = b.w; d.c[2][2] = c.x;
+		d.t[0][0] = c.y; d.t[0][1] = c.z;
+		d.t[1][0] = c.w; d.t[1][1] = e.x;
+		d.t[2][0] = e.y; d.t[2][1] = e.z;
+		d.material = __float_as_uint(e.w);
+	} else {
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			const uint32_t vi = s.d_vertex_indices[prim * 3u + k], ti = s.d_texcoord_indices[prim * 3u + k];
+#pragma unroll
+			for (int j = 0; j < 3; ++j) d.c[k][j] = s.d_vertices[(size_t)vi * 3 + j];
+			d.t[k][0] = s.d_texcoords[(size_t)ti * 2];
+			d.t[k][1] = s.d_texcoords[(size_t)ti * 2 + 1];
+		}
+		d.material = s.d_material_ids[prim];
+	}
+	return d;
+}
+
+// GetSceneVertex (Scene.glsl:50-53): (x, y, z, 1) times the instance's 3x4 rows
+__device__ __forceinline__ void transform_vertex(const float *T, const float *v, float *o) {
+#pragma unroll
+	for (int j = 0; j < 3; ++j) o[j] = T[4 * j] * v[0] + T[4 * j + 1] * v[1] + T[4 * j + 2] * v[2] + T[4 * j + 3];
+}
+
 // UnpackNRCInput (NRCRecord.glsl:98-125) into the 14 floats of an nrc_query
+template <bool PACKED>
 __device__ void unpack_input(const nrc_scene &s, const nrc_packed_input &p, const float *lut, float *q) {
 	const uint32_t prim = p.primitive_id, inst = p.flip_bit_instance_id & 0x7FFFFFFFu;
 	const bool flip = (p.flip_bit_instance_id >> 31) != 0u;
+	const PrimitiveData d = load_primitive<PACKED>(s, prim);
+	const float *T = s.d_transforms + (size_t)inst * 12;
 	float v0[3], v1[3], v2[3];
-	scene_vertex(s, inst, prim, 0, v0);
-	scene_vertex(s, inst, prim, 1, v1);
-	scene_vertex(s, inst, prim, 2, v2);
+	transform_vertex(T, d.c[0], v0);
+	transform_vertex(T, d.c[1], v1);
+	transform_vertex(T, d.c[2], v2);
 	const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
 	const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
 	float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
@@ -113,14 +146,10 @@ __device__ void unpack_input(const nrc_scene &s, const nrc_packed_input &p, cons
 	for (int j = 0; j < 3; ++j) q[j] = v0[j] * b[0] + v1[j] * b[1] + v2[j] * b[2];
 	unorm2x16(p.scattered_dir_2x16u, q + 3);
 	sph_encode(n[0], n[1], n[2], q + 5);
-	const nrc_material &m = s.d_materials[s.d_material_ids[prim]];
+	const nrc_material &m = s.d_materials[d.material];
 	q[7] = m.roughness;
-	const uint32_t t0 = s.d_texcoord_indices[prim * 3u], t1 = s.d_texcoord_indices[prim * 3u + 1],
-	               t2 = s.d_texcoord_indices[prim * 3u + 2];
-	const float u = s.d_texcoords[(size_t)t0 * 2] * b[0] + s.d_texcoords[(size_t)t1 * 2] * b[1] +
-	                s.d_texcoords[(size_t)t2 * 2] * b[2];
-	const float v = s.d_texcoords[(size_t)t0 * 2 + 1] * b[0] + s.d_texcoords[(size_t)t1 * 2 + 1] * b[1] +
-	                s.d_texcoords[(size_t)t2 * 2 + 1] * b[2];
+	const float u = d.t[0][0] * b[0] + d.t[1][0] * b[1] + d.t[2][0] * b[2];
+	const float v = d.t[0][1] * b[0] + d.t[1][1] * b[1] + d.t[2][1] * b[2];
 	material_color(s, m.diffuse, m.diffuse_texture_id, u, v, lut, q + 8);
 	material_color(s, m.specular, m.specular_texture_id, u, v, lut, q + 11);
 }
@@ -134,6 +163,7 @@ __device__ __forceinline__ nrc_packed_input load_packed(const uint32_t *w) {
 	return p;
 }
 
+template <bool PACKED>
 __global__ __launch_bounds__(256) void unpack_eval_kernel(const nrc_scene s, const nrc_eval_record *__restrict__ rec,
                                                           const uint32_t *d_count, uint32_t max_count,
                                                           float *__restrict__ out) {
@@ -143,12 +173,13 @@ __global__ __launch_bounds__(256) void unpack_eval_kernel(const nrc_scene s, con
 	if (i >= effective_count(d_count, max_count, max_count)) return;
 	const uint32_t *w = reinterpret_cast<const uint32_t *>(rec + i);
 	float q[kQueryFloats];
-	unpack_input(s, load_packed(w + 1), lut, q);
+	unpack_input<PACKED>(s, load_packed(w + 1), lut, q);
 	float2 *o = reinterpret_cast<float2 *>(out + (size_t)i * kQueryFloats);
 #pragma unroll
 	for (int k = 0; k < kQueryFloats / 2; ++k) o[k] = make_float2(q[2 * k], q[2 * k + 1]);
 }
 
+template <bool PACKED>
 __global__ __launch_bounds__(256) void unpack_train_kernel(const nrc_scene s,
                                                            const nrc_wire_train_record *__restrict__ rec,
                                                            const uint32_t *d_count, uint32_t max_count,
@@ -159,7 +190,7 @@ __global__ __launch_bounds__(256) void unpack_train_kernel(const nrc_scene s,
 	if (i >= effective_count(d_count, max_count, max_count)) return;
 	const uint32_t *w = reinterpret_cast<const uint32_t *>(rec + i);
 	float q[kTrainFloats];
-	unpack_input(s, load_packed(w + 6), lut, q);
+	unpack_input<PACKED>(s, load_packed(w + 6), lut, q);
 	// nrc_gradient.comp:30-33: the target is the record's bias
 	q[14] = __uint_as_float(w[0]);
 	q[15] = __uint_as_float(w[1]);
@@ -167,6 +198,21 @@ __global__ __launch_bounds__(256) void unpack_train_kernel(const nrc_scene s,
 	float *o = out + (size_t)i * kTrainFloats;
 #pragma unroll
 	for (int k = 0; k < kTrainFloats; ++k) o[k] = q[k];
+}
+
+// One thread per primitive: the corners and texture coordinates its indices name and
+// its material id, in the 64-byte record that load_primitive<true> reads (16-float
+// order: corners 0..2 xyz, texcoords 0..2 uv, material id bits).
+__global__ __launch_bounds__(256) void pack_primitives_kernel(const nrc_scene s, uint32_t n,
+                                                              nrc_scene_primitive *__restrict__ out) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= n) return;
+	const PrimitiveData d = load_primitive<false>(s, i);
+	float4 *o = reinterpret_cast<float4 *>(out + i);
+	o[0] = make_float4(d.c[0][0], d.c[0][1], d.c[0][2], d.c[1][0]);
+	o[1] = make_float4(d.c[1][1], d.c[1][2], d.c[2][0], d.c[2][1]);
+	o[2] = make_float4(d.c[2][2], d.t[0][0], d.t[0][1], d.t[1][0]);
+	o[3] = make_float4(d.t[1][1], d.t[2][0], d.t[2][1], __uint_as_float(d.material));
 }
 
 // nrc_inference.comp:50-72 over wire records: the dst code comes with the eval
@@ -300,17 +346,29 @@ __global__ __launch_bounds__(256) void screen_kernel(const float *__restrict__ c
 
 } // namespace
 
+hipError_t launch_pack_primitives(const nrc_scene &s, uint32_t n, nrc_scene_primitive *out, hipStream_t stream) {
+	if (n == 0) return hipSuccess;
+	pack_primitives_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(s, n, out);
+	return hipGetLastError();
+}
+
 hipError_t launch_unpack_eval(const nrc_scene &s, const nrc_eval_record *rec, const uint32_t *d_count,
                               uint32_t max_count, float *queries, hipStream_t stream) {
 	if (max_count == 0) return hipSuccess;
-	unpack_eval_kernel<<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, queries);
+	if (s.d_primitives)
+		unpack_eval_kernel<true><<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, queries);
+	else
+		unpack_eval_kernel<false><<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, queries);
 	return hipGetLastError();
 }
 
 hipError_t launch_unpack_train(const nrc_scene &s, const nrc_wire_train_record *rec, const uint32_t *d_count,
                                uint32_t max_count, float *out, hipStream_t stream) {
 	if (max_count == 0) return hipSuccess;
-	unpack_train_kernel<<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, out);
+	if (s.d_primitives)
+		unpack_train_kernel<true><<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, out);
+	else
+		unpack_train_kernel<false><<<dim3((max_count + 255) / 256), dim3(256), 0, stream>>>(s, rec, d_count, max_count, out);
 	return hipGetLastError();
 }
 

@@ -94,7 +94,8 @@ class Scene(C.Structure):
     """nrc_scene: device pointers of the scene buffers (Scene.glsl:26-47)."""
     _fields_ = [("d_vertices", C.c_void_p), ("d_vertex_indices", C.c_void_p), ("d_texcoords", C.c_void_p),
                 ("d_texcoord_indices", C.c_void_p), ("d_materials", C.c_void_p), ("d_material_ids", C.c_void_p),
-                ("d_transforms", C.c_void_p), ("d_textures", C.c_void_p), ("texture_count", C.c_uint32)]
+                ("d_transforms", C.c_void_p), ("d_textures", C.c_void_p), ("texture_count", C.c_uint32),
+                ("d_primitives", C.c_void_p)]
 
 
 class GroupItem(C.Structure):
@@ -148,6 +149,7 @@ def load_library():
     L.nrc_sph_encode.argtypes = [P, U32, P, P]
     L.nrc_mlp_forward_f16.argtypes = [U32, P, P, U32, P, P]
     SC = C.POINTER(Scene)
+    L.nrc_scene_pack_primitives.argtypes = [SC, U32, P, P]
     L.nrc_unpack_eval_records.argtypes = [SC, P, P, U32, P, P]
     L.nrc_unpack_train_records.argtypes = [SC, P, P, U32, P, P]
     L.nrc_composite_records.argtypes = [P, P, P, U32, C.POINTER(Screen), P, U32, P]
@@ -166,7 +168,8 @@ def load_library():
     for f in ("nrc_init_weights", "nrc_cache_create", "nrc_cache_reset", "nrc_cache_get_desc", "nrc_set_use_ema",
               "nrc_infer", "nrc_train", "nrc_frame", "nrc_get_state", "nrc_set_state", "nrc_encode",
               "nrc_sph_encode", "nrc_mlp_forward_f16", "nrc_mlp_gradient_f16", "nrc_mlp_gradient_f16_noalloc",
-              "nrc_group_infer", "nrc_unpack_eval_records", "nrc_unpack_train_records", "nrc_composite_records",
+              "nrc_group_infer", "nrc_scene_pack_primitives", "nrc_unpack_eval_records", "nrc_unpack_train_records",
+              "nrc_composite_records",
               "nrc_build_train_records", "nrc_screen_tonemap",
               "nrc_group_train", "nrc_group_frame", "nrc_group_frame_graph", "nrc_graph_launch"):
         getattr(L, f).restype = C.c_int
@@ -451,7 +454,25 @@ class DeviceScene:
         self.c = Scene(self.vertices.data_ptr(), self.vertex_indices.data_ptr(), self.texcoords.data_ptr(),
                        self.texcoord_indices.data_ptr(), self.materials.data_ptr(), self.material_ids.data_ptr(),
                        self.transforms.data_ptr(), self.texture_table.data_ptr() if textures else None,
-                       len(textures))
+                       len(textures), None)
+        self.torch = torch
+        self.primitive_count = int(np.asarray(material_ids).shape[0])
+        self.primitives = None
+
+    def pack_primitives(self, stream=None) -> None:
+        """Builds the 64-byte packed primitive records (nrc_scene_pack_primitives); later unpacks read
+        them."""
+        self.primitives = self.torch.empty(max(1, self.primitive_count) * 64, dtype=self.torch.uint8,
+                                           device=self.vertices.device)
+        _check(load_library().nrc_scene_pack_primitives(C.byref(self.c), self.primitive_count,
+                                                        self.primitives.data_ptr(), _stream(stream)))
+        self.c.d_primitives = self.primitives.data_ptr()
+
+    def use_packed(self, on: bool) -> None:
+        """Switches the unpack between the packed records (after pack_primitives) and the index arrays."""
+        if on and self.primitives is None:
+            raise RuntimeError("pack_primitives() first")
+        self.c.d_primitives = self.primitives.data_ptr() if on else None
 
 
 def unpack_eval_records(scene: DeviceScene, records, max_count: int, queries, count=None, stream=None) -> None:

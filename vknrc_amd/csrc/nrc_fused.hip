@@ -154,8 +154,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 						const uint32_t m = base + (tg + tt) * 16 + r;
 						if (m < count) {
 							float *dst = out + (size_t)m * kOut;
+							// the clamp as the pipelined kernel applies it: an integer max of the bits
+							// (sign bit set -> +0), so both kernels give the same bits
+							const int lo_bits = grp.clamp0 ? 0 : INT_MIN;
 #pragma unroll
-							for (int c = 0; c < kOut; ++c) dst[c] = grp.clamp0 ? fmaxf(o[tt][c], 0.0f) : o[tt][c];
+							for (int c = 0; c < kOut; ++c) {
+								const int b = __float_as_int(o[tt][c]);
+								dst[c] = __int_as_float(b > lo_bits ? b : lo_bits);
+							}
 						}
 					}
 				}

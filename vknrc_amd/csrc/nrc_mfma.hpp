@@ -84,7 +84,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 //   * one-blob (NRCOneBlob4Encode, NRCRecord.glsl:51-63): per knot k = 0..4,
 //     c_k = clamp(0.5 + u (15/16 - 5/8 u^2 + 3/16 u^4), 0, 1) with u = k - 4x,
 //     which is (15/16) u (1 - (2/3) u^2 + (1/5) u^4) + 0.5 in Horner form
-//     (6 instructions a knot instead of 11);
+//     (6 instructions a knot instead of 11; oneblob4_knots, which the FP8 kernel
+//     uses); the FP16 kernels evaluate only the two unsaturated knots and place
+//     the packed quad with byte permutes (oneblob4_packed);
 //   * SINCOS6: sin/cos(2^k pi p) for k = 0..5 from one polynomial evaluation at
 //     k = 0 and five double-angle steps (largest error 1.3e-5 at k = 5, a twentieth
 //     of an FP16 half-ulp at 1);

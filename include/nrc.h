@@ -333,7 +333,9 @@ typedef struct nrc_scene {
 /* Packs primitives [0, primitive_count) of the scene into d_out [primitive_count],
  * 64-byte aligned (device memory from the caller), once per scene: unpacking a record
  * then reads one 64-byte record for its primitive instead of about nine scattered
- * cache lines of indices, corners, texcoords and material ids.  No counterpart in the
+ * cache lines of indices, corners, texcoords and material ids.  primitive_count must
+ * not exceed the primitives the index arrays hold, and every index must name an
+ * existing vertex and texcoord (as the unpack itself assumes).  No counterpart in the
  * reference; for callers that unpack many records on one scene. */
 nrc_status nrc_scene_pack_primitives(const nrc_scene *scene, uint32_t primitive_count, nrc_scene_primitive *d_out,
                                      void *stream);

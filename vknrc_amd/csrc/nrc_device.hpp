@@ -222,6 +222,14 @@ template <int ENC> __device__ __forceinline__ void encode_all(const float *rec, 
 	}
 }
 
+// float -> int32 as v_cvt_i32_f32 converts: toward zero, saturating at the int32
+// range, NaN -> 0 (a C++ cast of a NaN or of an out-of-range value is undefined).
+__device__ __forceinline__ int cvt_i32_sat(float x) {
+	int k;
+	asm("v_cvt_i32_f32 %0, %1" : "=v"(k) : "v"(x));
+	return k;
+}
+
 // b + f * p with the product and the sum rounded separately: the empty asm makes
 // the product opaque, so the compiler cannot contract the pair into an FMA.
 __device__ __forceinline__ float mul_add_rn(float b, float f, float p) {

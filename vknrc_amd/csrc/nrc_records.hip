@@ -41,7 +41,7 @@ __device__ __forceinline__ void repeat_texels(float u, uint32_t n, uint32_t &i0,
 	const float x = uf * (float)n - 0.5f;
 	const float x0 = floorf(x);
 	a = x - x0;
-	const int k = (int)x0;
+	const int k = cvt_i32_sat(x0);
 	i0 = k < 0 ? n - 1u : (uint32_t)k < n ? (uint32_t)k : 0u;
 	i1 = (uint32_t)(k + 1) < n ? (uint32_t)(k + 1) : 0u;
 }

@@ -174,3 +174,21 @@ def test_python_mirrors_match_the_c_layout_of_the_header():
         assert n.MATERIAL.fields[f][1] == h[f"nrc_material.{f}"], f
     assert n.TRACED_PATH.itemsize == h["nrc_traced_path"]
     assert h["nrc_query"] == 56 and h["nrc_train_record"] == 68
+
+
+def test_scene_calls_validate_before_touching_a_device():
+    L = v.load_library()
+    s = v.nrc.Scene()                                   # every buffer NULL
+    assert L.nrc_scene_pack_primitives(C.byref(s), 10, C.c_void_p(4096), None) != 0
+    assert b"scene buffer NULL" in L.nrc_last_error()
+    assert L.nrc_scene_pack_primitives(None, 10, C.c_void_p(4096), None) != 0
+    assert L.nrc_scene_pack_primitives(C.byref(s), 0, None, None) == 0     # nothing to pack
+    fake = [C.c_void_p(256 * (i + 1)) for i in range(7)]
+    s2 = v.nrc.Scene(*[f.value for f in fake], None, 0, None)
+    assert L.nrc_scene_pack_primitives(C.byref(s2), 10, None, None) != 0
+    assert b"NULL buffer" in L.nrc_last_error()
+    assert L.nrc_scene_pack_primitives(C.byref(s2), 10, C.c_void_p(4096 + 16), None) != 0
+    assert b"64-byte aligned" in L.nrc_last_error()
+    s2.d_primitives = 4096 + 4                          # the unpack needs 16-byte aligned records
+    assert L.nrc_unpack_eval_records(C.byref(s2), C.c_void_p(4096), None, 10, C.c_void_p(8192), None) != 0
+    assert b"16-byte aligned" in L.nrc_last_error()

@@ -310,7 +310,8 @@ __global__ __launch_bounds__(256) void build_train_records_kernel(const nrc_trac
 // screen.frag:10-31
 __device__ __forceinline__ float filmic_hejl(float h) {
 	const float a = 1.425f * h + 0.05f;
-	return (h * a + 0.004f) / (h * (a + 0.55f) + 0.0491f) - 0.0821f;
+	// the quotient through the hardware reciprocal (about 1 ulp), as a shader computes it
+	return (h * a + 0.004f) * __builtin_amdgcn_rcpf(h * (a + 0.55f) + 0.0491f) - 0.0821f;
 }
 
 __global__ __launch_bounds__(256) void screen_kernel(const float *__restrict__ color, float *accumulate_img,
@@ -325,21 +326,24 @@ __global__ __launch_bounds__(256) void screen_kernel(const float *__restrict__ c
 		float4 *acc = reinterpret_cast<float4 *>(accumulate_img) + px;
 		if (count != 0u) {
 			const float4 a = *acc;
-			const float n = (float)count, n1 = (float)(count + 1u);
-			c[0] = (c[0] + a.x * n) / n1;
-			c[1] = (c[1] + a.y * n) / n1;
-			c[2] = (c[2] + a.z * n) / n1;
+			// one reciprocal for the three channels (within 2 ulps of the division)
+			const float n = (float)count, r = __builtin_amdgcn_rcpf((float)(count + 1u));
+			c[0] = (c[0] + a.x * n) * r;
+			c[1] = (c[1] + a.y * n) * r;
+			c[2] = (c[2] + a.z * n) * r;
 		}
 		*acc = make_float4(c[0], c[1], c[2], 0.0f);
 	}
 	// ToneMapFilmic_Hejl2015(color, 3.2): the curve of each channel over the curve of the white point
-	const float fw = filmic_hejl(3.2f);
+	const float rfw = __builtin_amdgcn_rcpf(filmic_hejl(3.2f));
 	float o[3];
 #pragma unroll
 	for (int k = 0; k < 3; ++k) {
-		// pow of the slightly negative values the curve gives near black is undefined in GLSL; 0 here
-		const float t = filmic_hejl(c[k]) / fw;
-		o[k] = powf(fmaxf(t, 0.0f), 1.0f / 2.2f);
+		const float t = filmic_hejl(c[k]) * rfw;
+		// GLSL pow(t, 1/2.2) is exp2(log2(t) / 2.2) (the precision GLSL specifies for it), here on the
+		// hardware log2 and exp2; pow of the slightly negative values the curve gives near black is
+		// undefined in GLSL, 0 here
+		o[k] = t > 0.0f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(t) * (1.0f / 2.2f)) : 0.0f;
 	}
 	reinterpret_cast<float4 *>(out)[px] = make_float4(o[0], o[1], o[2], 1.0f);
 }

@@ -169,13 +169,24 @@ nrc::AdamParams adam_params(const nrc_cache *c) {
 	return p;
 }
 
+// The plain path's scratch, allocated on its first use.  All three buffers are set
+// together or not at all, so a failed attempt leaves the cache as it was.
 nrc_status ensure_plain_scratch(nrc_cache *c) {
 	if (c->acts) return NRC_OK;
 	const uint32_t ldn = (c->desc.train_batch_size + 63u) & ~63u;
 	const size_t slab = (size_t)(c->desc.hidden_layers + 1) * 64 * ldn;
-	NRC_HIP(hipMalloc(&c->acts, slab * sizeof(float)), "plain scratch");
-	NRC_HIP(hipMalloc(&c->deltas, slab * sizeof(float)), "plain scratch");
-	NRC_HIP(hipMalloc(&c->sample_loss, (size_t)ldn * sizeof(float)), "plain scratch");
+	float *acts = nullptr, *deltas = nullptr, *loss = nullptr;
+	hipError_t e = hipMalloc(&acts, slab * sizeof(float));
+	if (e == hipSuccess) e = hipMalloc(&deltas, slab * sizeof(float));
+	if (e == hipSuccess) e = hipMalloc(&loss, (size_t)ldn * sizeof(float));
+	if (e != hipSuccess) {
+		for (float *p : {acts, deltas, loss})
+			if (p) (void)hipFree(p);
+		return hip_fail(e, "plain scratch");
+	}
+	c->acts = acts;
+	c->deltas = deltas;
+	c->sample_loss = loss;
 	c->ldn = ldn;
 	return NRC_OK;
 }
@@ -731,21 +742,25 @@ nrc_status nrc_mlp_gradient_f16(uint32_t hidden, const uint16_t *d_w, const uint
 	const size_t nf = nrc::fwd_frag_halves((int)hidden), nt = nrc::bwd_frag_halves((int)hidden);
 	float *partials = nullptr;
 	_Float16 *wf = nullptr, *wtf = nullptr;
-	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s), "gradient scratch");
-	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wf), 2 * nf, s), "gradient scratch");
-	NRC_HIP(hipMallocAsync(reinterpret_cast<void **>(&wtf), 2 * nt, s), "gradient scratch");
+	hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&partials), sizeof(float) * (size_t)pstride * nb, s);
+	if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void **>(&wf), 2 * nf, s);
+	if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void **>(&wtf), 2 * nt, s);
 	uint32_t nparts = 0;
 	const _Float16 *w = reinterpret_cast<const _Float16 *>(d_w);
-	hipError_t e = hipMemsetAsync(wf, 0, 2 * nf, s);
+	if (e == hipSuccess) e = hipMemsetAsync(wf, 0, 2 * nf, s);
 	if (e == hipSuccess) e = hipMemsetAsync(wtf, 0, 2 * nt, s);
 	if (e == hipSuccess) e = nrc::launch_make_frags((int)hidden, w, wf, wtf, s);
 	if (e == hipSuccess)
 		e = nrc::launch_fused_gradient_f16((int)hidden, reinterpret_cast<const _Float16 *>(d_in), d_targets, n, wf, wtf,
 		                                   partials, pstride, &nparts, s);
 	if (e == hipSuccess) e = nrc::launch_reduce_partials(partials, nparts, pstride, wc, d_dw, s);
-	hipError_t e2 = hipFreeAsync(partials, s);
-	if (e2 == hipSuccess) e2 = hipFreeAsync(wf, s);
-	if (e2 == hipSuccess) e2 = hipFreeAsync(wtf, s);
+	// every scratch buffer that was allocated is freed, whatever failed
+	hipError_t e2 = hipSuccess;
+	for (void *p : {static_cast<void *>(partials), static_cast<void *>(wf), static_cast<void *>(wtf)})
+		if (p) {
+			const hipError_t f = hipFreeAsync(p, s);
+			if (e2 == hipSuccess) e2 = f;
+		}
 	if (e != hipSuccess) return hip_fail(e, "nrc_mlp_gradient_f16");
 	if (e2 != hipSuccess) return hip_fail(e2, "nrc_mlp_gradient_f16 free");
 	return NRC_OK;

@@ -30,6 +30,12 @@ extern "C" {
 #define NRC_DEFAULT_TRAIN_BATCH_SIZE 16384
 /* Largest train batch one nrc_train call accepts. */
 #define NRC_MAX_TRAIN_BATCH_SIZE 65536
+/* Most queries or records one call takes: max_count of nrc_infer and the record
+ * and epilogue calls, max_queries of nrc_frame and the group calls, n of
+ * nrc_encode, nrc_sph_encode and nrc_mlp_forward_f16, n_paths and eval_capacity
+ * of nrc_build_train_records.  Counts are uint32 and the launches round them up to
+ * whole tiles and blocks; larger counts give NRC_ERROR_CAPACITY. */
+#define NRC_MAX_QUERIES 0x80000000u
 
 typedef enum nrc_status {
 	NRC_OK = 0,

@@ -192,3 +192,28 @@ def test_scene_calls_validate_before_touching_a_device():
     s2.d_primitives = 4096 + 4                          # the unpack needs 16-byte aligned records
     assert L.nrc_unpack_eval_records(C.byref(s2), C.c_void_p(4096), None, 10, C.c_void_p(8192), None) != 0
     assert b"16-byte aligned" in L.nrc_last_error()
+
+
+def test_counts_beyond_the_query_limit_are_refused_before_touching_a_device():
+    """Every call that takes a query or record count refuses counts above NRC_MAX_QUERIES with
+    NRC_ERROR_CAPACITY (5) before any launch: the launches round counts up to whole tiles and blocks
+    in 32 bits.  The buffers here are never dereferenced."""
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "nrc.h")).read()
+    limit = int(re.search(r"#define NRC_MAX_QUERIES (0x[0-9A-Fa-f]+)u", hdr).group(1), 16)
+    assert limit == v.nrc.MAX_QUERIES
+    L = v.load_library()
+    big = limit + 1
+    p = [C.c_void_p(4096 * (i + 1)) for i in range(8)]
+    assert L.nrc_composite(p[0], p[1], None, big, None, None, None, 0, None) == 5
+    assert L.nrc_encode(v.ENCODING_SINCOS6, p[0], big, p[1], None) == 5
+    assert L.nrc_sph_encode(p[0], big, p[1], None) == 5
+    assert L.nrc_mlp_forward_f16(5, p[0], p[1], big, p[2], None) == 5
+    assert L.nrc_composite_records(p[0], p[1], None, big, None, None, 0, None) == 5
+    assert L.nrc_build_train_records(p[0], big, p[1], p[2], 16384, p[3], p[4], 1, None) == 5
+    assert L.nrc_build_train_records(p[0], 1, p[1], p[2], 16384, p[3], p[4], big, None) == 5
+    scene = v.nrc.Scene(*[4096 * (i + 1) for i in range(7)], None, 0, None)
+    assert L.nrc_unpack_eval_records(C.byref(scene), p[0], None, big, p[1], None) == 5
+    assert L.nrc_unpack_train_records(C.byref(scene), p[0], None, big, p[1], None) == 5
+    assert b"NRC_MAX_QUERIES" in L.nrc_last_error()

@@ -55,6 +55,11 @@ nrc_status hip_fail(hipError_t e, const char *what) {
 		if (e_ != hipSuccess) return hip_fail(e_, what);                                                               \
 	} while (0)
 
+// A query or record count one call may take (NRC_MAX_QUERIES).
+nrc_status check_count(uint32_t n) {
+	return n > NRC_MAX_QUERIES ? fail(NRC_ERROR_CAPACITY, "count > NRC_MAX_QUERIES") : NRC_OK;
+}
+
 // SplitMix64 stream; uniform doubles in [0, 1) from the top 53 bits.
 struct SplitMix64 {
 	uint64_t s;
@@ -444,6 +449,7 @@ nrc_status nrc_infer(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d
 	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
 	if (max_count == 0) return NRC_OK;
 	if (!d_queries || !d_rgb) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_count) != NRC_OK) return NRC_ERROR_CAPACITY;
 	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 	const int H = (int)c->desc.hidden_layers, enc = (int)c->desc.encoding;
 	const float *q = reinterpret_cast<const float *>(d_queries);
@@ -511,6 +517,7 @@ nrc_status nrc_frame(nrc_cache *c, const nrc_query *d_queries, const uint32_t *d
                      uint32_t max_batch_records, nrc_path path, void *stream) {
 	if (!c) return fail(NRC_ERROR_INVALID_ARGUMENT, "cache is NULL");
 	if (max_queries && (!d_queries || !d_rgb)) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_queries) != NRC_OK) return NRC_ERROR_CAPACITY;
 	nrc_status st = max_batch_records ? check_train(c, d_batches, max_batch_records, path) : NRC_OK;
 	if (st != NRC_OK) return st;
 	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -560,6 +567,7 @@ nrc_status nrc_encode(nrc_encoding encoding, const nrc_query *d_queries, uint32_
 	if ((unsigned)encoding > NRC_ENCODING_UV_SINCOS16) return fail(NRC_ERROR_INVALID_ARGUMENT, "unknown encoding");
 	if (n == 0) return NRC_OK;
 	if (!d_queries || !d_features) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(n) != NRC_OK) return NRC_ERROR_CAPACITY;
 	hipError_t e = nrc::launch_encode((int)encoding, reinterpret_cast<const float *>(d_queries), n, d_features,
 	                                  reinterpret_cast<hipStream_t>(stream));
 	if (e != hipSuccess) return hip_fail(e, "nrc_encode");
@@ -569,6 +577,7 @@ nrc_status nrc_encode(nrc_encoding encoding, const nrc_query *d_queries, uint32_
 nrc_status nrc_sph_encode(const float *d_dirs, uint32_t n, float *d_uv, void *stream) {
 	if (n == 0) return NRC_OK;
 	if (!d_dirs || !d_uv) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(n) != NRC_OK) return NRC_ERROR_CAPACITY;
 	hipError_t e = nrc::launch_sph_encode(d_dirs, n, d_uv, reinterpret_cast<hipStream_t>(stream));
 	if (e != hipSuccess) return hip_fail(e, "nrc_sph_encode");
 	return NRC_OK;
@@ -585,6 +594,7 @@ nrc_status nrc_composite(const float *d_rgb, const uint32_t *d_dst, const uint32
                          uint32_t batch_stride, void *stream) {
 	if (max_count == 0) return NRC_OK;
 	if (!d_rgb || !d_dst) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_count) != NRC_OK) return NRC_ERROR_CAPACITY;
 	if (screen && (!screen->d_bias_factor_r || !screen->d_factor_gb || screen->pitch < screen->width))
 		return fail(NRC_ERROR_INVALID_ARGUMENT, "screen images NULL or pitch < width");
 	if (d_batches && (!d_train_factors || batch_stride == 0))
@@ -646,6 +656,7 @@ nrc_status nrc_unpack_eval_records(const nrc_scene *scene, const nrc_eval_record
 	nrc_status st = check_scene(scene);
 	if (st != NRC_OK) return st;
 	if (!d_records || !d_queries) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_count) != NRC_OK) return NRC_ERROR_CAPACITY;
 	hipError_t e = nrc::launch_unpack_eval(*scene, d_records, d_count, max_count, reinterpret_cast<float *>(d_queries),
 	                                       reinterpret_cast<hipStream_t>(stream));
 	if (e != hipSuccess) return hip_fail(e, "nrc_unpack_eval_records");
@@ -659,6 +670,7 @@ nrc_status nrc_unpack_train_records(const nrc_scene *scene, const nrc_wire_train
 	nrc_status st = check_scene(scene);
 	if (st != NRC_OK) return st;
 	if (!d_records || !d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_count) != NRC_OK) return NRC_ERROR_CAPACITY;
 	hipError_t e = nrc::launch_unpack_train(*scene, d_records, d_count, max_count, reinterpret_cast<float *>(d_out),
 	                                        reinterpret_cast<hipStream_t>(stream));
 	if (e != hipSuccess) return hip_fail(e, "nrc_unpack_train_records");
@@ -670,6 +682,7 @@ nrc_status nrc_composite_records(const float *d_rgb, const nrc_eval_record *d_re
                                  uint32_t batch_stride, void *stream) {
 	if (max_count == 0) return NRC_OK;
 	if (!d_rgb || !d_records) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(max_count) != NRC_OK) return NRC_ERROR_CAPACITY;
 	if (screen && (!screen->d_bias_factor_r || !screen->d_factor_gb || screen->pitch < screen->width))
 		return fail(NRC_ERROR_INVALID_ARGUMENT, "screen images NULL or pitch < width");
 	if (d_batches && batch_stride == 0) return fail(NRC_ERROR_INVALID_ARGUMENT, "batch_stride 0");
@@ -690,6 +703,7 @@ nrc_status nrc_build_train_records(const nrc_traced_path *d_paths, uint32_t n_pa
 	// the train code holds record indices in 14 bits (EncodeNRCEvalDstTrain, NRCRecord.glsl:20-22)
 	if (batch_capacity == 0 || batch_capacity > (1u << 14))
 		return fail(NRC_ERROR_INVALID_ARGUMENT, "batch_capacity must be in [1, 16384]");
+	if (check_count(n_paths) != NRC_OK || check_count(eval_capacity) != NRC_OK) return NRC_ERROR_CAPACITY;
 	hipError_t e = nrc::launch_build_train_records(d_paths, n_paths, d_batches, d_batch_counts, batch_capacity,
 	                                               d_eval_records, d_eval_count, eval_capacity,
 	                                               reinterpret_cast<hipStream_t>(stream));
@@ -714,6 +728,7 @@ nrc_status nrc_mlp_forward_f16(uint32_t hidden, const uint16_t *d_w, const uint1
 	if (hidden != 4 && hidden != 5) return fail(NRC_ERROR_INVALID_ARGUMENT, "hidden_layers must be 4 or 5");
 	if (n == 0) return NRC_OK;
 	if (!d_w || !d_in || !d_out) return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+	if (check_count(n) != NRC_OK) return NRC_ERROR_CAPACITY;
 	const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 	const size_t nf = nrc::fwd_frag_halves((int)hidden);
 	_Float16 *wf = nullptr;
@@ -914,9 +929,11 @@ nrc_status check_group_infer(const nrc_group_item *items, uint32_t n_items, nrc_
 	if (st != NRC_OK) return st;
 	if (path != NRC_PATH_PLAIN_F32 && path != NRC_PATH_FUSED_F16 && path != NRC_PATH_FUSED_F8)
 		return fail(NRC_ERROR_UNSUPPORTED, "unknown path");
-	for (uint32_t i = 0; i < n_items; ++i)
+	for (uint32_t i = 0; i < n_items; ++i) {
 		if (items[i].max_queries && (!items[i].d_queries || !items[i].d_rgb))
 			return fail(NRC_ERROR_INVALID_ARGUMENT, "NULL buffer");
+		if (check_count(items[i].max_queries) != NRC_OK) return NRC_ERROR_CAPACITY;
+	}
 	return NRC_OK;
 }
 

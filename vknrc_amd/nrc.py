@@ -28,6 +28,7 @@ TRAIN_RECORD_FLOATS = 17
 TRAIN_BATCH_COUNT = 4
 DEFAULT_TRAIN_BATCH_SIZE = 16384
 MAX_TRAIN_BATCH_SIZE = 65536
+MAX_QUERIES = 0x80000000  # most queries or records one call takes (NRC_MAX_QUERIES)
 
 PATH_PLAIN_F32, PATH_FUSED_F16, PATH_FUSED_F8 = 0, 1, 2
 ENCODING_TRI12, ENCODING_SINCOS6, ENCODING_UV_SINCOS16 = 0, 1, 2

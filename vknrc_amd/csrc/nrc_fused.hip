@@ -707,45 +707,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				S.go[c * LD + col] = Bg[c];
 			}
 		}
-		// ---- backward chain: delta_H = (W_H^T g_o) . 1[a_H > 0], then delta_l for l = H-1 .. 1
-		half8_t Bd[2];
-		{
-			float4_t acc[4];
-#pragma unroll
-			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
-			const uint4 mH[2] = {act_masks(B[0]), act_masks(B[1])}; // B still holds a_H
-			relu_bwd_pack(acc, mH, Bd);
-			store_bcol_pairs<LD>(S.dlt[H - 1], Bd, g, col);
-		}
-#pragma unroll
-		for (int l = H - 1; l >= 1; --l) {
-			// the layer's eight W^T fragments are read from LDS together, so the
-			// MFMAs wait on one LDS round trip rather than on one each
-			half8_t wb[4][2];
-#pragma unroll
-			for (int c = 0; c < 2; ++c)
-#pragma unroll
-				for (int mt = 0; mt < 4; ++mt) wb[mt][c] = wt_frag(((l - 1) * 4 + mt) * 2 + c);
-			__builtin_amdgcn_sched_barrier(0);
-			float4_t acc[4];
-#pragma unroll
-			for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
-#pragma unroll
-			for (int c = 0; c < 2; ++c)
-#pragma unroll
-				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wb[mt][c], Bd[c], acc[mt]);
-			relu_bwd_pack(acc, keep[l], Bd);
-			store_bcol_pairs<LD>(S.dlt[l - 1], Bd, g, col);
-		}
-		// this wave's loss sum, in a fixed order
-#pragma unroll
-		for (int off = 32; off >= 1; off >>= 1) lv += __shfl_xor(lv, off);
-		if (lane == 0) S.lsum[wave] += lv;
-		block_lds_sync();
-
-		// ---- dW_l^T tiles of this wave over the chunk's records (K = the 64 records)
-#pragma unroll
-		for (int l = 0; l < H; ++l) {
+		// dW_l^T tiles of this wave over the chunk's records (K = the 64 records, all
+		// waves' a_l and delta): summed into accW by a multi-chunk workgroup, stored at
+		// once by a one-chunk workgroup
+		auto dw_layer = [&](int l) __attribute__((always_inline)) {
 			float4_t t[2][2];
 #pragma unroll
 			for (int i = 0; i < 2; ++i) t[i][0] = t[i][1] = (float4_t)0.0f;
@@ -772,21 +737,75 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 					else
 						store_tile(l, it0 + i, ot0 + o2, t[i][o2]);
 				}
-		}
+		};
 		// dW_H^T: input tile `wave` against the output-gradient rows
-		{
+		auto dw_output = [&]() __attribute__((always_inline)) {
 			float4_t t = (float4_t)0.0f;
 #pragma unroll
 			for (int kc = 0; kc < R / 32; ++kc) {
-				const half8_t a =
-				    *reinterpret_cast<const half8_t *>(S.act[H] + (16 * wave + r) * LD + 32 * kc + 8 * g);
+				const half8_t a = *reinterpret_cast<const half8_t *>(S.act[H] + (16 * wave + r) * LD + 32 * kc + 8 * g);
 				const half8_t b = *reinterpret_cast<const half8_t *>(S.go + r * LD + 32 * kc + 8 * g);
 				t = mfma(a, b, t);
 			}
 			if (undo != 1.0f) t *= undo;
 			accO += t;
+		};
+		// this wave's loss sum, in a fixed order
+#pragma unroll
+		for (int off = 32; off >= 1; off >>= 1) lv += __shfl_xor(lv, off);
+		if (lane == 0) S.lsum[wave] += lv;
+
+		// ---- backward chain: delta_H = (W_H^T g_o) . 1[a_H > 0], then delta_l for l = H-1 .. 1.
+		// A one-chunk workgroup takes each layer's dW as soon as every wave has stored
+		// that layer's delta, so its partial-row stores drain while the backward runs
+		// instead of all at the end; a multi-chunk workgroup, which stores once after
+		// its last chunk, takes them all after the backward.
+		half8_t Bd[2];
+		{
+			float4_t acc[4];
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
+			const uint4 mH[2] = {act_masks(B[0]), act_masks(B[1])}; // B still holds a_H
+			relu_bwd_pack(acc, mH, Bd);
+			store_bcol_pairs<LD>(S.dlt[H - 1], Bd, g, col);
+		}
+		if constexpr (!MULTI) {
+			block_lds_sync(); // delta_H and the output gradients of every wave
+			dw_layer(H - 1);
+			dw_output();
+		}
+#pragma unroll
+		for (int l = H - 1; l >= 1; --l) {
+			// the layer's eight W^T fragments are read from LDS together, so the
+			// MFMAs wait on one LDS round trip rather than on one each
+			half8_t wb[4][2];
+#pragma unroll
+			for (int c = 0; c < 2; ++c)
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt) wb[mt][c] = wt_frag(((l - 1) * 4 + mt) * 2 + c);
+			__builtin_amdgcn_sched_barrier(0);
+			float4_t acc[4];
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) acc[mt] = (float4_t)0.0f;
+#pragma unroll
+			for (int c = 0; c < 2; ++c)
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wb[mt][c], Bd[c], acc[mt]);
+			relu_bwd_pack(acc, keep[l], Bd);
+			store_bcol_pairs<LD>(S.dlt[l - 1], Bd, g, col);
+			if constexpr (!MULTI) {
+				block_lds_sync(); // delta_l of every wave
+				dw_layer(l - 1);
+			}
+		}
+		if constexpr (MULTI) {
+			block_lds_sync(); // every wave's deltas (and loss sum)
+#pragma unroll
+			for (int l = 0; l < H; ++l) dw_layer(l);
+			dw_output();
 		}
 	}
+	if constexpr (!MULTI) block_lds_sync(); // every wave's loss sum
 
 	// ---- the workgroup's partial row: dW tiles (float4 per lane) and the loss sum
 	if constexpr (MULTI) {

@@ -23,9 +23,12 @@ def code_blocks(lang):
     return re.findall(r"^```" + lang + r"\n(.*?)^```", text, flags=re.S | re.M)
 
 
-def test_cpp_wrapper_compiles_and_links(tmp_path):
+@pytest.mark.parametrize("compiler", ["hipcc", "g++"])
+def test_cpp_wrapper_compiles_and_links(tmp_path, compiler):
     if HIPCC is None:
-        pytest.skip("hipcc is not installed")
+        pytest.skip("ROCm is not installed")
+    if compiler == "g++" and shutil.which("g++") is None:
+        pytest.skip("g++ is not installed")
     blocks = code_blocks("cpp")
     assert len(blocks) == 1
     (tmp_path / "HipNRC.hpp").write_text(blocks[0])
@@ -38,9 +41,13 @@ def test_cpp_wrapper_compiles_and_links(tmp_path):
     (tmp_path / "main.cpp").write_text(
         '#include "HipNRC.hpp"\nint main(int argc, char **) {\n' + takes +
         "\tif (argc > 1000) { HipNRC n(1); }\n\treturn 0;\n}\n")
-    cmd = [HIPCC, "-Wall", "-Werror", "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include"),
-           "-I" + str(tmp_path), str(tmp_path / "main.cpp"), "-L" + os.path.join(ROOT, "vknrc_amd"), "-lvknrc",
-           "-o", str(tmp_path / "main")]
+    if compiler == "hipcc":
+        cmd = [HIPCC, "-Wno-unused-command-line-argument"]
+    else:  # a host compiler needs the platform macro and include path for hipStream_t
+        rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+        cmd = ["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(rocm, "include")]
+    cmd += ["-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), "-I" + str(tmp_path),
+            str(tmp_path / "main.cpp"), "-L" + os.path.join(ROOT, "vknrc_amd"), "-lvknrc", "-o", str(tmp_path / "main")]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-4000:]
 

@@ -363,15 +363,20 @@ def test_nonfinite_gradient_semantics():
     assert s3["state"]["t"] == 1 and s3["state"]["loss_scale"] == 64.0 and s3["state"]["clean_steps"] == 1
 
 
-def test_large_finite_gradients_are_scaled_not_skipped(o64):
+@pytest.mark.parametrize("chunks", [0, 4])
+def test_large_finite_gradients_are_scaled_not_skipped(o64, chunks):
     """An output gradient beyond FP16 range (2 * 128 * 3e5 / 0.01 here) puts the
     workgroup's deltas under a power-of-two scale instead of overflowing: the fused
-    step is taken and matches the oracle like any other."""
+    step is taken and matches the oracle like any other.  With 4 chunks a
+    workgroup, record 3 scales the first chunk of workgroup 0 (three unscaled
+    chunks follow) and record 2500 the last chunk of workgroup 9 (after three
+    unscaled ones), so the dW sums carried across chunks are rescaled both ways."""
     rng = np.random.default_rng(15)
     r = train_records(3000, 16)
     r[3, 14:17] = 3e5
     r[2500, 14:17] = 1e6
     st, w = make_cache(rng, loss_scale=128.0, dynamic_loss_scale=True)
+    st.set_grad_chunks(chunks)
     st.train(dev(r), path=v.PATH_FUSED_F16)
     s = st.get_state()
     assert s["state"]["t"] == 1 and s["state"]["skipped_steps"] == 0 and s["state"]["loss_scale"] == 128.0

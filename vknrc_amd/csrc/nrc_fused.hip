@@ -707,13 +707,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				S.go[c * LD + col] = Bg[c];
 			}
 		}
+		// A multi-chunk workgroup sums its dW tiles in the MFMA accumulators
+		// themselves.  For a chunk whose deltas carry a scale 2^k, the sums are scaled
+		// by 2^k before its dW and by 2^-k after (exact: powers of two), so the
+		// scale costs nothing on the chunks that have none.
+		auto scale_sums = [&](float s) __attribute__((always_inline)) {
+			// kept behind its branch: the compiler would otherwise multiply on every
+			// chunk, since x * 1 is x
+			asm volatile("" ::: "memory");
+#pragma unroll
+			for (int l = 0; l < (MULTI ? H : 1); ++l)
+#pragma unroll
+				for (int i = 0; i < 2; ++i)
+#pragma unroll
+					for (int o2 = 0; o2 < 2; ++o2) accW[l][i][o2] *= s;
+			accO *= s;
+		};
+		if constexpr (MULTI)
+			if (undo != 1.0f) scale_sums(dscale); // uniform
 		// dW_l^T tiles of this wave over the chunk's records (K = the 64 records, all
-		// waves' a_l and delta): summed into accW by a multi-chunk workgroup, stored at
-		// once by a one-chunk workgroup
+		// waves' a_l and delta): accumulated into accW by a multi-chunk workgroup,
+		// stored at once by a one-chunk workgroup
 		auto dw_layer = [&](int l) __attribute__((always_inline)) {
 			float4_t t[2][2];
 #pragma unroll
-			for (int i = 0; i < 2; ++i) t[i][0] = t[i][1] = (float4_t)0.0f;
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int o2 = 0; o2 < 2; ++o2) t[i][o2] = MULTI ? accW[MULTI ? l : 0][i][o2] : (float4_t)0.0f;
 #pragma unroll
 			for (int kc = 0; kc < R / 32; ++kc) {
 				half8_t a[2], b[2];
@@ -731,24 +751,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
 				for (int o2 = 0; o2 < 2; ++o2) {
-					if (undo != 1.0f) t[i][o2] *= undo; // uniform; exact when taken
-					if constexpr (MULTI)
-						accW[MULTI ? l : 0][i][o2] += t[i][o2];
-					else
+					if constexpr (MULTI) {
+						accW[MULTI ? l : 0][i][o2] = t[i][o2];
+					} else {
+						if (undo != 1.0f) t[i][o2] *= undo; // uniform; exact when taken
 						store_tile(l, it0 + i, ot0 + o2, t[i][o2]);
+					}
 				}
 		};
 		// dW_H^T: input tile `wave` against the output-gradient rows
 		auto dw_output = [&]() __attribute__((always_inline)) {
-			float4_t t = (float4_t)0.0f;
+			float4_t t = MULTI ? accO : (float4_t)0.0f;
 #pragma unroll
 			for (int kc = 0; kc < R / 32; ++kc) {
 				const half8_t a = *reinterpret_cast<const half8_t *>(S.act[H] + (16 * wave + r) * LD + 32 * kc + 8 * g);
 				const half8_t b = *reinterpret_cast<const half8_t *>(S.go + r * LD + 32 * kc + 8 * g);
 				t = mfma(a, b, t);
 			}
-			if (undo != 1.0f) t *= undo;
-			accO += t;
+			if constexpr (MULTI) {
+				accO = t;
+			} else {
+				if (undo != 1.0f) t *= undo;
+				accO += t;
+			}
 		};
 		// this wave's loss sum, in a fixed order
 #pragma unroll
@@ -803,6 +828,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 			for (int l = 0; l < H; ++l) dw_layer(l);
 			dw_output();
+			if (undo != 1.0f) scale_sums(undo); // uniform
 		}
 	}
 	if constexpr (!MULTI) block_lds_sync(); // every wave's loss sum

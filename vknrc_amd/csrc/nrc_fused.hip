@@ -747,16 +747,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 					for (int o2 = 0; o2 < 2; ++o2) t[i][o2] = mfma(a[i], b[o2], t[i][o2]);
 			}
+			if constexpr (!MULTI)
+				if (undo != 1.0f) { // uniform; exact when taken
+					asm volatile("" ::: "memory"); // keeps the multiplies behind the branch
+#pragma unroll
+					for (int i = 0; i < 2; ++i)
+#pragma unroll
+						for (int o2 = 0; o2 < 2; ++o2) t[i][o2] *= undo;
+				}
 #pragma unroll
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
 				for (int o2 = 0; o2 < 2; ++o2) {
-					if constexpr (MULTI) {
+					if constexpr (MULTI)
 						accW[MULTI ? l : 0][i][o2] = t[i][o2];
-					} else {
-						if (undo != 1.0f) t[i][o2] *= undo; // uniform; exact when taken
+					else
 						store_tile(l, it0 + i, ot0 + o2, t[i][o2]);
-					}
 				}
 		};
 		// dW_H^T: input tile `wave` against the output-gradient rows
@@ -771,7 +777,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			if constexpr (MULTI) {
 				accO = t;
 			} else {
-				if (undo != 1.0f) t *= undo;
+				if (undo != 1.0f) {
+					asm volatile("" ::: "memory");
+					t *= undo;
+				}
 				accO += t;
 			}
 		};

@@ -341,20 +341,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 // forward A fragments (W_l) sit in registers, 168 for H = 5, loaded once from the
 // fragment-major copy (through LDS in a one-chunk workgroup); the backward ones
 // (W_l^T) are copied once per workgroup into LDS with direct global->LDS loads.
-// Waves write their activations a_l and deltas feature-major into LDS as they
+// Waves write their activations a_l and deltas record-major into LDS as they
 // produce them, so the forward and the backward chain run without any barrier
 // between waves; the backward takes its ReLU masks from registers.  After the
 // one barrier, wave w computes its 2x2 block of dW_l^T tiles (input tiles
 // 2(w&1).., output tiles 2(w>>1)..) for every layer from LDS, K = the 64 records;
 // a multi-chunk workgroup sums them in registers over its chunks, and the tiles
 // go (float4 per lane, non-temporal) to the workgroup's partial-gradient row.
-constexpr int kSmallLd = (int)kFusedSmallRecords + 8; // halves per LDS row
+//
+// LDS rows are records: a lane stores its B operand (its record's 16 features of
+// the operand, two runs of 4 consecutive features a K-chunk) with plain 8-byte
+// stores, and the dW operands, which need 8 consecutive records of one feature a
+// lane, come back through transposing reads (ds_read_b64_tr_b16).
+constexpr int kRowLd = kWidth + 16; // halves per record row: the 4 rows of one transposing read start 8 banks apart
+constexpr int kGoLd = 16;           // halves per output-gradient row (units 3..15 stay zero)
 
 template <int H> struct SmallGradSmem {
-	_Float16 wt[H * 8 * 512];                // backward A fragments, fragment-major as in HBM
-	_Float16 act[H + 1][kWidth * kSmallLd]; // a_l, [feature][record]
-	_Float16 dlt[H][kWidth * kSmallLd];     // delta_{l+1} = dL/dz at layer l's output, [unit][record]
-	_Float16 go[16 * kSmallLd];             // output gradient rows 0..2; rows 3..15 stay zero
+	_Float16 wt[H * 8 * 512];                               // backward A fragments, fragment-major as in HBM
+	_Float16 act[H + 1][kFusedSmallRecords * kRowLd];       // a_l, [record][feature]
+	_Float16 dlt[H][kFusedSmallRecords * kRowLd];           // delta_{l+1} = dL/dz at layer l's output, [record][unit]
+	_Float16 go[kFusedSmallRecords * kGoLd];                // output gradient, [record][unit]
 	float lsum[4];
 	float gmax[4]; // largest |output gradient| of each wave's records
 };
@@ -364,42 +370,42 @@ template <int H> struct SmallGradSmem {
 // FP16 maximum for the growth of the deltas through the layers.
 constexpr float kDeltaLimit = 8192.0f;
 
-// writes this lane's 16 values of a B operand (record column col) feature-major, row stride LD
-template <int LD> __device__ __forceinline__ void store_bcol_ld(_Float16 *dst, const half8_t *b, int g, int col) {
-#pragma unroll
-	for (int c = 0; c < 2; ++c)
-#pragma unroll
-		for (int j = 0; j < 8; ++j) dst[bfeat(c, g, j) * LD + col] = b[c][j];
-}
-
-// The same store as dword pairs: lanes 2m and 2m+1 hold the same 16 features of
-// records col and col+1, so one DPP swap of each half2 register with the
-// neighbour lane and one byte permute give the even lane feature f_2k of both
-// records and the odd lane feature f_2k+1 of both; each lane then writes one
-// dword where store_bcol_ld writes two halves (the LDS layout is the same).
-template <int LD> __device__ __forceinline__ void store_bcol_pairs(_Float16 *dst, const half8_t *b, int g, int col) {
-	const uint32_t odd = (uint32_t)col & 1u;
-	const uint32_t sel = odd ? 0x03020706u : 0x05040100u; // odd: (X.hi, V.hi); even: (V.lo, X.lo)
+// writes this lane's B operand (record col: features bfeat(c, g, 0..3) =
+// 32c + 4g + 0..3 and bfeat(c, g, 4..7) = 32c + 16 + 4g + 0..3) into its row
+__device__ __forceinline__ void store_brow(_Float16 *rows, const half8_t *b, int g, int col) {
+	_Float16 *p = rows + col * kRowLd + 4 * g;
 #pragma unroll
 	for (int c = 0; c < 2; ++c) {
-		const uint4 v4 = __builtin_bit_cast(uint4, b[c]);
-		const uint32_t v[4] = {v4.x, v4.y, v4.z, v4.w};
-#pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			const uint32_t x = (uint32_t)__builtin_amdgcn_mov_dpp((int)v[k], 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
-			const uint32_t w = __builtin_amdgcn_perm(x, v[k], sel);
-			const int f = bfeat(c, g, 2 * k) + (int)odd;
-			*reinterpret_cast<uint32_t *>(dst + f * LD + (col - (int)odd)) = w;
-		}
+		*reinterpret_cast<half4_t *>(p + 32 * c) = b[c].lo;
+		*reinterpret_cast<half4_t *>(p + 32 * c + 16) = b[c].hi;
 	}
 }
 
-// reads back this lane's 16 values of a B operand (record column col) written by store_bcol_ld
-template <int LD> __device__ __forceinline__ void load_bcol_ld(const _Float16 *src, half8_t *b, int g, int col) {
+// reads back this lane's B operand (record col) written by store_brow
+__device__ __forceinline__ void load_brow(const _Float16 *rows, half8_t *b, int g, int col) {
+	const _Float16 *p = rows + col * kRowLd + 4 * g;
 #pragma unroll
-	for (int c = 0; c < 2; ++c)
-#pragma unroll
-		for (int j = 0; j < 8; ++j) b[c][j] = src[bfeat(c, g, j) * LD + col];
+	for (int c = 0; c < 2; ++c) {
+		const half4_t lo = *reinterpret_cast<const half4_t *>(p + 32 * c);
+		const half4_t hi = *reinterpret_cast<const half4_t *>(p + 32 * c + 16);
+		b[c] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+	}
+}
+
+// A K-operand of a dW MFMA (K = records) from record-major rows of stride LDR:
+// lane (g, r) gets X[record 32 kc + 8 g + k][feature f0 + r], k = 0..7.
+// ds_read_b64_tr_b16 gives lane i of each 16-lane group element i % 4 of the 4
+// halves that lane 4 k + i / 4 of the group addresses, as element k (k = 0..3);
+// so lane j of group g addresses features f0 + 4 (j & 3) .. + 3 of record
+// 32 kc + 8 g + (j >> 2), and again 4 records on for elements 4..7.
+template <int LDR>
+__device__ __forceinline__ half8_t load_kop(const _Float16 *rows, int f0, int kc, int g, int r) {
+	typedef short short4_t __attribute__((ext_vector_type(4)));
+	typedef __attribute__((address_space(3))) short4_t lds_short4;
+	const _Float16 *p = rows + (32 * kc + 8 * g + (r >> 2)) * LDR + f0 + 4 * (r & 3);
+	const half4_t lo = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)p));
+	const half4_t hi = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)(p + 4 * LDR)));
+	return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
@@ -468,7 +474,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	uint32_t *__restrict__ overflow = gi.overflow;
 	constexpr int WC = weight_count(H);
 	constexpr int R = (int)kFusedSmallRecords;
-	constexpr int LD = kSmallLd;
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
 	SmallGradSmem<H> &S = *reinterpret_cast<SmallGradSmem<H> *>(smem_raw);
 
@@ -582,9 +587,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		return *reinterpret_cast<const half8_t *>(S.wt + blk * 512 + lane * 8);
 	};
 
-	// rows 3..15 of the output-gradient operand stay zero; the loss sums start at zero
-	for (int i = tid; i < 13 * 32; i += 256)
-		reinterpret_cast<uint32_t *>(S.go + (3 + i / 32) * LD)[i % 32] = 0u;
+	// units 3..15 of the output-gradient rows stay zero; the loss sums start at zero
+	for (int i = tid; i < R * kGoLd / 2; i += 256) reinterpret_cast<uint32_t *>(S.go)[i] = 0u;
 	if (tid < 4) S.lsum[tid] = 0.0f;
 
 	float4_t accW[MULTI ? H : 1][2][2], accO = (float4_t)0.0f; // dW tiles summed over the chunks
@@ -607,22 +611,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		}
 		// ---- encode: wave w computes feature block w (features 16w ..) of record base + lane
 		{
-			_Float16 *a0 = S.act[0] + lane;
+			_Float16 *a0 = S.act[0] + lane * kRowLd;
 			const bool valid = base + lane < count;
 			if constexpr (F16IN) {
 #pragma unroll
 				for (int h = 0; h < 2; ++h) {
 					const half8_t v = __builtin_bit_cast(half8_t, fq[h]);
-#pragma unroll
-					for (int j = 0; j < 8; ++j) a0[(16 * wave + 8 * h + j) * LD] = valid ? v[j] : (_Float16)0.0f;
+					*reinterpret_cast<half8_t *>(a0 + 16 * wave + 8 * h) = valid ? v : (half8_t)0;
 				}
 			} else {
 				auto put = [&](auto qc) __attribute__((always_inline)) {
 					constexpr int qb = decltype(qc)::value;
 					float f[16];
 					encode_block<ENC>(q, qb, f);
+					half8_t h[2];
 #pragma unroll
-					for (int j = 0; j < 16; ++j) a0[(16 * qb + j) * LD] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+					for (int j = 0; j < 16; ++j) h[j >> 3][j & 7] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+					*reinterpret_cast<half8_t *>(a0 + 16 * qb) = h[0];
+					*reinterpret_cast<half8_t *>(a0 + 16 * qb + 8) = h[1];
 				};
 				switch (wave) {
 				case 0: put(std::integral_constant<int, 0>{}); break;
@@ -639,12 +645,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		if constexpr (MULTI)
 			if (ch + 1 < nch) load_inputs(base + R); // the next chunk's inputs (uniform branch)
 
-		// ---- forward for this wave's 16 records; a_l kept as B operands and written feature-major
+		// ---- forward for this wave's 16 records; a_l kept as B operands and written to its record rows
 		// (the activations are not kept in registers for the backward: its ReLU masks
 		// come back from the LDS copy, which leaves room for the weight fragments)
 		half8_t B[2];
 		uint4 keep[H][2]; // ReLU masks of a_1 .. a_{H-1} in B layout, for the backward
-		load_bcol_ld<LD>(S.act[0], B, g, col);
+		load_brow(S.act[0], B, g, col);
 #pragma unroll
 		for (int l = 0; l < H; ++l) {
 			float4_t acc[4];
@@ -660,7 +666,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				keep[l + 1][0] = act_masks(B[0]);
 				keep[l + 1][1] = act_masks(B[1]);
 			}
-			store_bcol_pairs<LD>(S.act[l + 1], B, g, col);
+			store_brow(S.act[l + 1], B, g, col);
 		}
 		float4_t o = (float4_t)0.0f;
 #pragma unroll
@@ -704,7 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 			for (int c = 0; c < kOut; ++c) {
 				Bg[c] = (_Float16)(go[c] * dscale);
-				S.go[c * LD + col] = Bg[c];
+				S.go[col * kGoLd + c] = Bg[c];
 			}
 		}
 		// A multi-chunk workgroup sums its dW tiles in the MFMA accumulators
@@ -739,8 +745,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				half8_t a[2], b[2];
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
-					a[i] = *reinterpret_cast<const half8_t *>(S.act[l] + (16 * (it0 + i) + r) * LD + 32 * kc + 8 * g);
-					b[i] = *reinterpret_cast<const half8_t *>(S.dlt[l] + (16 * (ot0 + i) + r) * LD + 32 * kc + 8 * g);
+					a[i] = load_kop<kRowLd>(S.act[l], 16 * (it0 + i), kc, g, r);
+					b[i] = load_kop<kRowLd>(S.dlt[l], 16 * (ot0 + i), kc, g, r);
 				}
 #pragma unroll
 				for (int i = 0; i < 2; ++i)
@@ -770,8 +776,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			float4_t t = MULTI ? accO : (float4_t)0.0f;
 #pragma unroll
 			for (int kc = 0; kc < R / 32; ++kc) {
-				const half8_t a = *reinterpret_cast<const half8_t *>(S.act[H] + (16 * wave + r) * LD + 32 * kc + 8 * g);
-				const half8_t b = *reinterpret_cast<const half8_t *>(S.go + r * LD + 32 * kc + 8 * g);
+				const half8_t a = load_kop<kRowLd>(S.act[H], 16 * wave, kc, g, r);
+				const half8_t b = load_kop<kGoLd>(S.go, 0, kc, g, r);
 				t = mfma(a, b, t);
 			}
 			if constexpr (MULTI) {
@@ -801,7 +807,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wt_frag(((H - 1) * 4 + mt) * 2), Bg, (float4_t)0.0f);
 			const uint4 mH[2] = {act_masks(B[0]), act_masks(B[1])}; // B still holds a_H
 			relu_bwd_pack(acc, mH, Bd);
-			store_bcol_pairs<LD>(S.dlt[H - 1], Bd, g, col);
+			store_brow(S.dlt[H - 1], Bd, g, col);
 		}
 		if constexpr (!MULTI) {
 			block_lds_sync(); // delta_H and the output gradients of every wave
@@ -826,7 +832,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(wb[mt][c], Bd[c], acc[mt]);
 			relu_bwd_pack(acc, keep[l], Bd);
-			store_bcol_pairs<LD>(S.dlt[l - 1], Bd, g, col);
+			store_brow(S.dlt[l - 1], Bd, g, col);
 			if constexpr (!MULTI) {
 				block_lds_sync(); // delta_l of every wave
 				dw_layer(l - 1);

@@ -7,6 +7,7 @@
 // fp16 weights of layer l, row-major [out][in], at l*4096; the 3x64 output
 // layer at H*4096 (padded to 16 rows with zeros inside the MFMA, NN_nv.glsl:77-82).
 // The kernels read the fragment-major copies of nrc_device.hpp.
+#include <cstddef>
 #include <cstdlib>
 #include <type_traits>
 
@@ -352,18 +353,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void f
 // LDS rows are records: a lane stores its B operand (its record's 16 features of
 // the operand, two runs of 4 consecutive features a K-chunk) with plain 8-byte
 // stores, and the dW operands, which need 8 consecutive records of one feature a
-// lane, come back through transposing reads (ds_read_b64_tr_b16).
-constexpr int kRowLd = kWidth + 16; // halves per record row: the 4 rows of one transposing read start 8 banks apart
-constexpr int kGoLd = 16;           // halves per output-gradient row (units 3..15 stay zero)
+// lane, come back through transposing reads (ds_read_b64_tr_b16).  A row is 64
+// halves with no padding; its 16 four-half chunks are permuted by a function of
+// the record (row_swizzle) under which the 8-byte row stores of 16 consecutive
+// records and the transposing reads of 4 records x 4 chunks hit distinct banks
+// (with 32 banks and 16 lanes a pass, and with 64 banks and 32 lanes a pass).
+constexpr int kRowLd = kWidth; // halves per record row
+constexpr int kGoLd = 16;      // halves per output-gradient row (units 3..15 are zero)
 
 template <int H> struct SmallGradSmem {
-	_Float16 wt[H * 8 * 512];                               // backward A fragments, fragment-major as in HBM
-	_Float16 act[H + 1][kFusedSmallRecords * kRowLd];       // a_l, [record][feature]
-	_Float16 dlt[H][kFusedSmallRecords * kRowLd];           // delta_{l+1} = dL/dz at layer l's output, [record][unit]
-	_Float16 go[kFusedSmallRecords * kGoLd];                // output gradient, [record][unit]
+	_Float16 wt[H * 8 * 512];                         // backward A fragments, fragment-major as in HBM
+	_Float16 act[H + 1][kFusedSmallRecords * kRowLd]; // a_l, [record][feature], chunks swizzled
+	_Float16 dlt[H][kFusedSmallRecords * kRowLd];     // delta_{l+1} = dL/dz at layer l's output, [record][unit], swizzled
+	_Float16 go[kFusedSmallRecords * kGoLd];          // output gradient, [record][unit]
 	float lsum[4];
 	float gmax[4]; // largest |output gradient| of each wave's records
 };
+
+// chunk ch (features 4ch .. 4ch+3) of record rec is chunk ch ^ row_swizzle(rec) of its
+// row: bits 0, 1 of rec & 15 as they are, bit 2 = rec bit 1 ^ bit 2, bit 3 = rec bit 0 ^ bit 3
+__device__ __forceinline__ int row_swizzle(int rec) {
+	rec &= 15;
+	return rec ^ ((rec & 2) << 1) ^ ((rec & 1) << 3);
+}
+// halves offset of features feat .. feat+3 (feat a multiple of 4) of record rec
+__device__ __forceinline__ int row_at(int rec, int feat) { return rec * kRowLd + (((feat >> 2) ^ row_swizzle(rec)) << 2); }
 
 // Output gradients at least this large (2^13) get a power-of-two scale that
 // brings the workgroup's largest one below it, leaving 8x headroom under the
@@ -373,38 +387,38 @@ constexpr float kDeltaLimit = 8192.0f;
 // writes this lane's B operand (record col: features bfeat(c, g, 0..3) =
 // 32c + 4g + 0..3 and bfeat(c, g, 4..7) = 32c + 16 + 4g + 0..3) into its row
 __device__ __forceinline__ void store_brow(_Float16 *rows, const half8_t *b, int g, int col) {
-	_Float16 *p = rows + col * kRowLd + 4 * g;
 #pragma unroll
 	for (int c = 0; c < 2; ++c) {
-		*reinterpret_cast<half4_t *>(p + 32 * c) = b[c].lo;
-		*reinterpret_cast<half4_t *>(p + 32 * c + 16) = b[c].hi;
+		*reinterpret_cast<half4_t *>(rows + row_at(col, 32 * c + 4 * g)) = b[c].lo;
+		*reinterpret_cast<half4_t *>(rows + row_at(col, 32 * c + 16 + 4 * g)) = b[c].hi;
 	}
 }
 
 // reads back this lane's B operand (record col) written by store_brow
 __device__ __forceinline__ void load_brow(const _Float16 *rows, half8_t *b, int g, int col) {
-	const _Float16 *p = rows + col * kRowLd + 4 * g;
 #pragma unroll
 	for (int c = 0; c < 2; ++c) {
-		const half4_t lo = *reinterpret_cast<const half4_t *>(p + 32 * c);
-		const half4_t hi = *reinterpret_cast<const half4_t *>(p + 32 * c + 16);
+		const half4_t lo = *reinterpret_cast<const half4_t *>(rows + row_at(col, 32 * c + 4 * g));
+		const half4_t hi = *reinterpret_cast<const half4_t *>(rows + row_at(col, 32 * c + 16 + 4 * g));
 		b[c] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 	}
 }
 
-// A K-operand of a dW MFMA (K = records) from record-major rows of stride LDR:
-// lane (g, r) gets X[record 32 kc + 8 g + k][feature f0 + r], k = 0..7.
-// ds_read_b64_tr_b16 gives lane i of each 16-lane group element i % 4 of the 4
-// halves that lane 4 k + i / 4 of the group addresses, as element k (k = 0..3);
-// so lane j of group g addresses features f0 + 4 (j & 3) .. + 3 of record
+// A K-operand of a dW MFMA (K = records): lane (g, r) gets X[record 32 kc + 8 g + k]
+// [feature f0 + r], k = 0..7, from the swizzled record rows (SWZ) or from plain rows
+// of stride LDR.  ds_read_b64_tr_b16 gives lane i of each 16-lane group element i % 4
+// of the 4 halves that lane 4 k + i / 4 of the group addresses, as element k
+// (k = 0..3); so lane j of group g addresses features f0 + 4 (j & 3) .. + 3 of record
 // 32 kc + 8 g + (j >> 2), and again 4 records on for elements 4..7.
-template <int LDR>
+template <bool SWZ, int LDR>
 __device__ __forceinline__ half8_t load_kop(const _Float16 *rows, int f0, int kc, int g, int r) {
 	typedef short short4_t __attribute__((ext_vector_type(4)));
 	typedef __attribute__((address_space(3))) short4_t lds_short4;
-	const _Float16 *p = rows + (32 * kc + 8 * g + (r >> 2)) * LDR + f0 + 4 * (r & 3);
-	const half4_t lo = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)p));
-	const half4_t hi = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)(p + 4 * LDR)));
+	const int rec = 32 * kc + 8 * g + (r >> 2), feat = f0 + 4 * (r & 3);
+	const _Float16 *p0 = rows + (SWZ ? row_at(rec, feat) : rec * LDR + feat);
+	const _Float16 *p1 = rows + (SWZ ? row_at(rec + 4, feat) : (rec + 4) * LDR + feat);
+	const half4_t lo = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)p0));
+	const half4_t hi = __builtin_bit_cast(half4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4 *)p1));
 	return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
@@ -557,12 +571,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		__builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(wtf + (k * 64 + lane) * 8),
 		                                 (__attribute__((address_space(3))) void *)(S.wt + k * 512), 16, 0, 0);
 	// The forward fragments stay in registers.  A one-chunk workgroup fetches them
-	// once into the delta buffers (first written by the backward, which every wave
-	// reaches after its forward has consumed them) and each wave reads them from
+	// once into the delta buffers and the output-gradient rows after them (first
+	// written by the loss and the backward, past the barrier that every wave reaches
+	// after its forward has consumed the fragments) and each wave reads them from
 	// there after the first barrier, so the workgroup reads them from L2 once
 	// rather than once a wave; a multi-chunk workgroup, which holds them across its
 	// chunks, loads them straight into registers.
-	static_assert(sizeof(S.dlt) >= (H * 8 + 2) * 1024, "the fragment staging area is the delta buffers");
+	static_assert(offsetof(SmallGradSmem<H>, go) == offsetof(SmallGradSmem<H>, dlt) + sizeof(S.dlt) &&
+	                  sizeof(S.dlt) + sizeof(S.go) >= (H * 8 + 2) * 1024,
+	              "the fragment staging area is the delta buffers and the output-gradient rows");
 	_Float16 *const stage = &S.dlt[0][0];
 	const half8_t *wfv = reinterpret_cast<const half8_t *>(wf);
 	half8_t Af[H][4][2], Ao[2];
@@ -587,8 +604,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		return *reinterpret_cast<const half8_t *>(S.wt + blk * 512 + lane * 8);
 	};
 
-	// units 3..15 of the output-gradient rows stay zero; the loss sums start at zero
-	for (int i = tid; i < R * kGoLd / 2; i += 256) reinterpret_cast<uint32_t *>(S.go)[i] = 0u;
+	// the loss sums start at zero
 	if (tid < 4) S.lsum[tid] = 0.0f;
 
 	float4_t accW[MULTI ? H : 1][2][2], accO = (float4_t)0.0f; // dW tiles summed over the chunks
@@ -611,24 +627,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 		}
 		// ---- encode: wave w computes feature block w (features 16w ..) of record base + lane
 		{
-			_Float16 *a0 = S.act[0] + lane * kRowLd;
+			_Float16 *a0 = S.act[0];
 			const bool valid = base + lane < count;
 			if constexpr (F16IN) {
 #pragma unroll
 				for (int h = 0; h < 2; ++h) {
-					const half8_t v = __builtin_bit_cast(half8_t, fq[h]);
-					*reinterpret_cast<half8_t *>(a0 + 16 * wave + 8 * h) = valid ? v : (half8_t)0;
+					const half8_t v = valid ? __builtin_bit_cast(half8_t, fq[h]) : (half8_t)0;
+					*reinterpret_cast<half4_t *>(a0 + row_at(lane, 16 * wave + 8 * h)) = v.lo;
+					*reinterpret_cast<half4_t *>(a0 + row_at(lane, 16 * wave + 8 * h + 4)) = v.hi;
 				}
 			} else {
 				auto put = [&](auto qc) __attribute__((always_inline)) {
 					constexpr int qb = decltype(qc)::value;
 					float f[16];
 					encode_block<ENC>(q, qb, f);
-					half8_t h[2];
+					half4_t h[4];
 #pragma unroll
-					for (int j = 0; j < 16; ++j) h[j >> 3][j & 7] = valid ? (_Float16)f[j] : (_Float16)0.0f;
-					*reinterpret_cast<half8_t *>(a0 + 16 * qb) = h[0];
-					*reinterpret_cast<half8_t *>(a0 + 16 * qb + 8) = h[1];
+					for (int j = 0; j < 16; ++j) h[j >> 2][j & 3] = valid ? (_Float16)f[j] : (_Float16)0.0f;
+#pragma unroll
+					for (int k = 0; k < 4; ++k) *reinterpret_cast<half4_t *>(a0 + row_at(lane, 16 * qb + 4 * k)) = h[k];
 				};
 				switch (wave) {
 				case 0: put(std::integral_constant<int, 0>{}); break;
@@ -706,12 +723,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			}
 		}
 		half8_t Bg = (half8_t)0;
-		if (g == 0) {
+		if (g == 0) { // the record's whole row: units 0..2, then zeros
 #pragma unroll
-			for (int c = 0; c < kOut; ++c) {
-				Bg[c] = (_Float16)(go[c] * dscale);
-				S.go[col * kGoLd + c] = Bg[c];
-			}
+			for (int c = 0; c < kOut; ++c) Bg[c] = (_Float16)(go[c] * dscale);
+			*reinterpret_cast<half8_t *>(S.go + col * kGoLd) = Bg;
+			*reinterpret_cast<half8_t *>(S.go + col * kGoLd + 8) = (half8_t)0;
 		}
 		// A multi-chunk workgroup sums its dW tiles in the MFMA accumulators
 		// themselves.  For a chunk whose deltas carry a scale 2^k, the sums are scaled
@@ -745,8 +761,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 				half8_t a[2], b[2];
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
-					a[i] = load_kop<kRowLd>(S.act[l], 16 * (it0 + i), kc, g, r);
-					b[i] = load_kop<kRowLd>(S.dlt[l], 16 * (ot0 + i), kc, g, r);
+					a[i] = load_kop<true, kRowLd>(S.act[l], 16 * (it0 + i), kc, g, r);
+					b[i] = load_kop<true, kRowLd>(S.dlt[l], 16 * (ot0 + i), kc, g, r);
 				}
 #pragma unroll
 				for (int i = 0; i < 2; ++i)
@@ -776,8 +792,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			float4_t t = MULTI ? accO : (float4_t)0.0f;
 #pragma unroll
 			for (int kc = 0; kc < R / 32; ++kc) {
-				const half8_t a = load_kop<kRowLd>(S.act[H], 16 * wave, kc, g, r);
-				const half8_t b = load_kop<kGoLd>(S.go, 0, kc, g, r);
+				const half8_t a = load_kop<true, kRowLd>(S.act[H], 16 * wave, kc, g, r);
+				const half8_t b = load_kop<false, kGoLd>(S.go, 0, kc, g, r);
 				t = mfma(a, b, t);
 			}
 			if constexpr (MULTI) {

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
 	SmallGradSmem<H> &S = *reinterpret_cast<SmallGradSmem<H> *>(smem_raw);
 
-	const int tid = threadIdx.x, wave = tid >> 6;
+	const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6); // wave-uniform: in an SGPR
 	int lane = tid & 63;
 	int g = lane >> 4, r = lane & 15;
 	int col = 16 * wave + r; // record column this lane runs through the network

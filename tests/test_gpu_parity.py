@@ -196,9 +196,10 @@ def train_records(n, seed, enc=v.ENCODING_TRI12):
     return r
 
 
-def one_step(o64, path, hidden, enc, loss, n, seed):
+def one_step(o64, path, hidden, enc, loss, n, seed, chunks=0):
     rng = np.random.default_rng(seed)
     st, w = make_cache(rng, hidden_layers=hidden, encoding=enc, loss=loss)
+    st.set_grad_chunks(chunks)
     r = train_records(n, seed, enc)
     st.train(dev(r), path=path)
     s = st.get_state()
@@ -239,6 +240,20 @@ def test_fused_train_step_matches_oracle(o64, hidden, enc, loss):
     assert rel_err(g, g_ref) < 5e-2
     assert abs(s["state"]["last_loss"] - l_ref) < 1e-2 * abs(l_ref)
     assert s["state"]["t"] == 1
+
+
+@pytest.mark.parametrize("hidden,enc,loss", [(5, v.ENCODING_TRI12, v.LOSS_RELATIVE_L2_LUMINANCE),
+                                             (4, v.ENCODING_UV_SINCOS16, v.LOSS_L2),
+                                             (4, v.ENCODING_TRI12, v.LOSS_RELATIVE_L2_LUMINANCE),
+                                             (5, v.ENCODING_SINCOS6, v.LOSS_L2)])
+def test_fused_multi_chunk_step_matches_oracle_for_each_network_kind(o64, hidden, enc, loss):
+    """4 chunks a workgroup over 3,000 records (12 workgroups, the last with 3 chunks, the
+    last chunk partial): the multi-chunk kernel of each depth, encoding and loss."""
+    st, w, r, s, g, g_ref, l_ref = one_step(o64, v.PATH_FUSED_F16, hidden, enc, loss, 3000, 51 + hidden + enc, chunks=4)
+    assert cosine(g, g_ref) > GRAD_COSINE_MIN, cosine(g, g_ref)
+    assert rel_err(g, g_ref) < 5e-2
+    assert abs(s["state"]["last_loss"] - l_ref) < 1e-2 * abs(l_ref)
+    assert s["state"]["t"] == 1 and s["state"]["last_count"] == 3000
 
 
 @pytest.mark.parametrize("n", [40000, 65536])
